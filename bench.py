@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3D FDFD solve path on one MI355X.
+
+Metric: solves/s to 1e-8 relative residual on the 128^3 workload (eps = 2.085 + 40
+random dielectric boxes, 10-cell SC-PML on all faces, Gaussian Jy sheet at x = 20),
+plus p50/p99 solve latency and the operator-apply HBM bandwidth.
+
+A step is one complete solve (complex64 COCG with FP64 residual replacement, stopped
+when the FP64 true residual ||b - A x|| / ||b|| <= 1e-8) with the operator, materials
+and right-hand side already resident in HBM.  Beside it: the same solve with plain
+FP64 COCG (same GPU), the field error of the mixed solve against an FP64 solve to
+1e-11, the apply kernel's achieved bandwidth timed with HIP events on its own stream,
+and the oracle (restated reference: CSR assembly + scipy qmr, complex128) timed on
+this host over a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# scipy qmr iterations the oracle needs on the 128^3 workload to reach rtol 1e-8,
+# measured by running the oracle to convergence on a CPU host
+QMR_ITERS_128 = None
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--size', type=int, default=128)
+    ap.add_argument('--rtol', type=float, default=1e-8)
+    ap.add_argument('--apply-reps', type=int, default=200)
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-iters', type=int, default=20)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, iters):
+    """Oracle on this host: CSR assembly of the scaled operator + `iters` qmr iterations."""
+    import scipy.sparse.linalg as spla
+    from oracle import fdfd_ref as ref
+    t0 = time.perf_counter()
+    A0 = ref.e_full(w['omega'], w['dxes'], w['epsilon'])
+    Pl, Pr = ref.e_full_preconditioners(w['dxes'])
+    A = (Pl @ A0 @ Pr).tocsr()
+    b = Pl @ (-1j * w['omega'] * w['J'])
+    t1 = time.perf_counter()
+    spla.qmr(A, b, rtol=1e-30, atol=0.0, maxiter=iters)
+    t2 = time.perf_counter()
+    per_iter = (t2 - t1) / iters
+    out = dict(assembly_s=t1 - t0, qmr_s_per_iter=per_iter, nnz=int(A.nnz))
+    if QMR_ITERS_128 is not None and w['shape'] == (128, 128, 128):
+        proj = (t1 - t0) + per_iter * QMR_ITERS_128
+        out.update(projected_solve_s=proj, value=1.0 / proj, qmr_iterations_to_1e8=QMR_ITERS_128)
+    return out
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        print(f'bench.py measures one GPU only: --gpus {args.gpus} / WORLD_SIZE {world} is not supported '
+              '(this build has no multi-GPU path; see DESIGN.md for how the solve would shard)')
+        sys.exit(2)
+
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    dev = torch.device('cuda:0')
+    w = workloads.boxes_128(args.size)
+    log(f'workload {w["name"]} shape {w["shape"]}')
+    op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
+    J = torch.from_numpy(w['J']).to(dev)
+    b = op.map(_lib.MX_MAP_RHS, J, coef=-1j * w['omega'])
+    x = torch.zeros_like(b)
+    work = {m: torch.empty(op.work_bytes(m), dtype=torch.uint8, device=dev) for m in (_lib.MX_MIXED, _lib.MX_FP64)}
+    maxiter = 200000
+
+    def solve(mode, rtol, xout):
+        xout.zero_()
+        return solve_batch([op], [dict(op=0, b=b, x=xout, rtol=rtol, atol=0.0, maxiter=maxiter, work=work[mode])],
+                           mode=mode)[0]
+
+    # ---- mixed-precision solves: the measured step
+    for i in range(args.warmup):
+        st = solve(_lib.MX_MIXED, args.rtol, x)
+        log(f'warmup {i}: {st}')
+    torch.cuda.synchronize()
+    lat = []
+    stats = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        st = solve(_lib.MX_MIXED, args.rtol, x)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+        stats.append(st)
+        log(f'step {i}: {lat[-1] * 1e3:.1f} ms {st}')
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    E_mixed = op.map(_lib.MX_MAP_E, x)
+    ok = all(s['status'] == 'converged' and s['rel_residual'] <= args.rtol for s in stats)
+
+    # ---- same-GPU FP64 COCG and the field-error reference
+    extra = {}
+    if not args.no_fp64:
+        x64 = torch.zeros_like(b)
+        t0 = time.perf_counter()
+        st64 = solve(_lib.MX_FP64, args.rtol, x64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        log(f'fp64 cocg: {t64 * 1e3:.1f} ms {st64}')
+        st_ref = solve(_lib.MX_FP64, 1e-11, x64)
+        E_ref = op.map(_lib.MX_MAP_E, x64)
+        ferr = float(torch.linalg.vector_norm(E_mixed - E_ref) / torch.linalg.vector_norm(E_ref))
+        log(f'fp64 reference solve (1e-11): {st_ref}; field error of mixed solve {ferr:.3e}')
+        extra.update(fp64_cocg=dict(solves_per_s=1.0 / t64, ms=t64 * 1e3, iterations=st64['iterations'],
+                                    rel_residual=st64['rel_residual']),
+                     mixed_over_fp64_speedup=(1.0 / numpy.median(lat)) / (1.0 / t64),
+                     field_error_vs_fp64_1e-11=ferr)
+        del x64
+
+    # ---- apply kernel roofline: HIP events on the stream the kernel is launched on
+    g = torch.Generator(device=dev).manual_seed(0)
+    xa = torch.randn(op.n3, dtype=torch.complex64, device=dev, generator=g)
+    ya = torch.empty_like(xa)
+    plan = op.plan('c64')
+    stream = torch.cuda.current_stream()
+    for _ in range(10):
+        plan.apply(xa, ya, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.apply_reps):
+        plan.apply(xa, ya, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    apply_us = e0.elapsed_time(e1) * 1e3 / args.apply_reps
+    bpc = op.apply_bytes_per_cell(8)
+    apply_bytes = bpc * op.ncell
+    achieved = apply_bytes / (apply_us * 1e-6) / 1e9
+    log(f'c64 apply: {apply_us:.2f} us, {bpc} B/cell, {achieved:.0f} GB/s ({plan.blocks} blocks)')
+    # complex128 apply for reference
+    xb = xa.to(torch.complex128)
+    yb = torch.empty_like(xb)
+    plan64 = op.plan('c128')
+    for _ in range(5):
+        plan64.apply(xb, yb, stream=stream)
+    e0.record(stream)
+    for _ in range(args.apply_reps // 2):
+        plan64.apply(xb, yb, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    apply64_us = e0.elapsed_time(e1) * 1e3 / (args.apply_reps // 2)
+    bpc64 = op.apply_bytes_per_cell(16)
+    extra['apply_c128'] = dict(us=apply64_us, bytes_per_cell=bpc64,
+                               gbs=bpc64 * op.ncell / (apply64_us * 1e-6) / 1e9)
+
+    # ---- CPU baseline (oracle) on a bounded sample, rank 0 only
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        cb = cpu_baseline(w, args.cpu_iters)
+        log(f'cpu baseline sample: {cb} ({time.perf_counter() - t0:.1f} s)')
+        cores = len(os.sched_getaffinity(0))
+        cpu = dict(value=cb.get('value'), unit='solves/s', cores=1, kind='port',
+                   sample=(f'oracle (restated reference: CSR e_full + Pl/Pr, scipy {__import__("scipy").__version__} '
+                           f'qmr, complex128) on the {w["shape"]} workload: CSR assembly {cb["assembly_s"]:.1f} s + '
+                           f'{args.cpu_iters} qmr iterations at {cb["qmr_s_per_iter"]:.3f} s/iteration'
+                           + (f', projected to {cb["qmr_iterations_to_1e8"]} iterations (full solve to 1e-8)'
+                              if 'value' in cb else ', no projection (iteration count unknown)')
+                           + f'; scipy CSR mat-vec is single-threaded, host has {cores} cores'),
+                   assembly_s=cb['assembly_s'], s_per_iteration=cb['qmr_s_per_iter'])
+
+    ms = elapsed / args.steps * 1e3
+    line = {
+        'metric': 'solves/s to 1e-8 residual @128^3 (+p50/p99 latency; op-apply HBM GB/s)',
+        'value': args.steps / elapsed,
+        'unit': 'solves/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'c64 inner + f64 residual/accumulation',
+        'data': 'synthetic (seeded box dielectrics, Gaussian sheet source)',
+        'config': {'workload': f'{w["name"]}: {w["shape"][0]}x{w["shape"][1]}x{w["shape"][2]} Yee grid, '
+                               '10-cell SC-PML, 40 dielectric boxes, Gaussian Jy sheet, rtol 1e-8',
+                   'parallelism': 'single GPU'},
+        'latency_ms': {'p50': float(numpy.percentile(lat, 50) * 1e3), 'p99': float(numpy.percentile(lat, 99) * 1e3)},
+        'converged': ok,
+        'iterations': stats[-1]['iterations'], 'replacements': stats[-1]['replacements'],
+        'rel_residual': stats[-1]['rel_residual'],
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'k_apply<float> (complex64 operator apply)', 'us_per_launch': apply_us,
+                     'bytes_per_cell': bpc, 'cells': op.ncell},
+        'cpu_baseline': cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,166 @@
+/*
+ * meanas_fdfd.h -- C ABI of the MI355X (gfx950) FDFD solve library libmeanas_fdfd.so.
+ *
+ * Drop-in boundary: meanas.fdfd.solvers.generic(omega, dxes, J, epsilon, mu, *, pec,
+ * pmc, adjoint, matrix_solver, matrix_solver_opts, E_guess) -> E
+ * (reference meanas/fdfd/solvers.py:66-141).  The Python package meanas_amd.fdfd.solvers
+ * binds these entry points with ctypes; a C/C++ caller can bind them directly.
+ *
+ * Conventions
+ *  - Every field vector is complex, interleaved (re, im), in the C-order layout
+ *    (3, nx, ny, nz) with z fastest: meanas.fdmath.vec (fdmath/vectorization.py:51-68).
+ *  - All pointers inside the descriptors and all vector arguments are DEVICE pointers
+ *    owned by the caller; the library keeps no reference past the call (plans and
+ *    sessions keep the descriptors they were created with until destroyed).
+ *  - hipStream_t arguments are passed as void*; NULL is the default stream.
+ *  - Return value: 0 on success, < 0 on error; mx_last_error() describes the last
+ *    error of the calling thread.  Entry points are not reentrant on one plan/session.
+ *
+ * The operator: A = Pl e_full Pr (solvers.py:118-125) with p factored per axis
+ * (p_x = sqrt(dxe_x[i]) sqrt(dxh_y[j]) sqrt(dxh_z[k]), ...), equal to the reference's
+ * sqrt(p^2) (fdfd/operators.py:116-120) up to a per-cell sign that cancels in E.
+ * The per-axis coefficient table holds, for each axis a and index n,
+ *   fa = 1/(sqrt(dxe_a[n]) sqrt(dxh_a[(n+1) mod N_a])),  fb = 1/(sqrt(dxe_a[n]) sqrt(dxh_a[n])),
+ *   se = sqrt(dxe_a[n]),  sh = sqrt(dxh_a[n])
+ * as 4 consecutive complex values, x axis entries first, then y, then z.
+ */
+#ifndef MEANAS_FDFD_H
+#define MEANAS_FDFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MX_ABI_VERSION 1
+
+/* precision / mode */
+#define MX_MIXED 0   /* complex64 COCG + FP64 residual replacement, FP64 solution */
+#define MX_FP64 1    /* complex128 COCG throughout */
+
+/* operator flags (mx_operator.flags) */
+#define MX_EPS_COMPLEX 1
+#define MX_EPS_ANISO 2   /* eps holds 3N values (per component); else N (isotropic) */
+#define MX_MU 4          /* inv_mu present (1/mu); else mu = 1 */
+#define MX_MU_COMPLEX 8
+#define MX_MU_ANISO 16
+#define MX_MASK 32       /* mask: one byte per cell, bits 0-2 PEC x,y,z, bits 3-5 PMC x,y,z */
+
+/* solve status (mx_stats.status) */
+#define MX_CONVERGED 0
+#define MX_MAXITER 1
+#define MX_BREAKDOWN 2   /* repeated COCG breakdown, stopped at the iteration limit */
+#define MX_NONFINITE 3
+#define MX_STALLED 4     /* FP64 residual replacements stopped reducing the true residual */
+
+/* One operator (one structure).  Replaces the sparse matrix that
+ * meanas/fdfd/operators.py:40-96 (e_full) and :99-123 (e_full_preconditioners)
+ * assemble; consumed matrix-free.  Device pointers; the *32 fields feed the complex64
+ * kernels and the *64 fields the complex128 kernels. */
+typedef struct mx_operator {
+  int32_t nx, ny, nz;
+  int32_t flags;
+  double omega2_re, omega2_im;  /* omega**2 */
+  const void* coef32;           /* (nx+ny+nz) x 4 complex64  (fa, fb, se, sh) */
+  const void* coef64;           /* (nx+ny+nz) x 4 complex128 */
+  const void* eps32;            /* float32 or complex64, N or 3N */
+  const void* eps64;            /* float64 or complex128 */
+  const void* inv_mu32;         /* NULL unless MX_MU */
+  const void* inv_mu64;
+  const uint8_t* mask;          /* NULL unless MX_MASK */
+} mx_operator;
+
+/* One linear system A x = b (one right-hand side of one operator). */
+typedef struct mx_system {
+  int32_t op;        /* index into the operator array of the call */
+  int32_t maxiter;   /* inner iteration limit (reference default: 10 * 3N, scipy qmr) */
+  double rtol, atol; /* stop when ||b - A x|| <= max(rtol ||b||, atol) (scaled system) */
+  const void* b;     /* complex128 3N, scaled right-hand side */
+  void* x;           /* complex128 3N: in = initial guess (scaled), out = solution (scaled) */
+  void* work;        /* mx_work_bytes(op, mode) bytes of device memory */
+} mx_system;
+
+typedef struct mx_options {
+  int32_t mode;        /* MX_MIXED or MX_FP64 */
+  int32_t tile_rows;   /* apply tile height: 2, 4 or 8 (0 = automatic) */
+  int32_t slab;        /* x-planes per apply block (0 = automatic) */
+  int32_t chunk;       /* iterations per captured graph (0 = automatic) */
+  int32_t check_every; /* iterations between residual-replacement checks (0 = automatic) */
+  int32_t use_graph;   /* 1: replay hipGraph-captured iteration chunks, 0: plain launches */
+  double delta;        /* replacement trigger: recursive ||r|| < delta * max ||r|| (0 = automatic) */
+  int32_t stall_iters; /* force an FP64 residual replacement after this many iterations without one
+                          (0 = automatic: 1000; < 0 = never) */
+  int32_t stall_limit; /* stop with MX_STALLED after this many consecutive replacements that do not
+                          halve the true residual (0 = automatic: 4; < 0 = never) */
+} mx_options;
+
+typedef struct mx_stats {
+  int32_t status;        /* MX_CONVERGED, ... */
+  int32_t iterations;    /* inner COCG iterations */
+  int32_t replacements;  /* FP64 residual replacements (incl. the initial residual) */
+  int32_t flags;         /* raw device state flags */
+  double bnorm;          /* ||b|| */
+  double rel_residual;   /* ||b - A x|| / ||b|| in FP64 at exit (scaled system) */
+} mx_stats;
+
+int mx_abi_version(void);
+const char* mx_last_error(void);
+
+/* Device bytes of per-system workspace (Krylov vectors, partial sums, state). */
+size_t mx_work_bytes(const mx_operator* op, int32_t mode);
+
+/* y = A x for one operator; x, y complex64 (mode MX_MIXED) or complex128 (MX_FP64).
+ * Replaces the CSR product A @ x that scipy performs inside qmr
+ * (scipy sparse/linalg/_isolve/iterative.py:996, called from fdfd/solvers.py:62). */
+int mx_apply(const mx_operator* op, int32_t mode, const void* x, void* y, void* stream);
+
+/* Reusable apply plan (tables built once; no device allocation per apply). */
+typedef struct mx_plan mx_plan;
+mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, int32_t slab);
+int mx_plan_apply(mx_plan* plan, const void* x, void* y, void* stream);
+int mx_plan_blocks(const mx_plan* plan);
+void mx_plan_destroy(mx_plan* plan);
+
+/* r = b - A x in FP64 (complex128 in and out); *norm2_out (host) = ||r||^2.
+ * The residual the reference's logging callback prints (fdfd/solvers.py:46-48). */
+int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream);
+
+/* Diagonal maps of generic() (fdfd/solvers.py:115,120-139), complex128 in/out:
+ *   MX_MAP_RHS      out = p * (c * in)              b = Pl (-i w J): c = -i w
+ *   MX_MAP_RHS_ADJ  out = conj(c * in) / p          conj of Pr^H (-i w J) (adjoint)
+ *   MX_MAP_E        out = in / p                    E = Pr x
+ *   MX_MAP_E_ADJ    out = conj(p * in)              E = Pl^H z, z = conj(x) (adjoint)
+ * With c = 1, MX_MAP_RHS / MX_MAP_RHS_ADJ also scale an initial guess E_guess. */
+#define MX_MAP_RHS 0
+#define MX_MAP_RHS_ADJ 1
+#define MX_MAP_E 2
+#define MX_MAP_E_ADJ 3
+int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const void* in, void* out, void* stream);
+
+/* Solve a batch of systems (one or many right-hand sides, one or many operators of
+ * uneven size) in one set of launches per iteration.  Replaces the qmr call of
+ * fdfd/solvers.py:62 (scipy sparse/linalg/_isolve/iterative.py:844-1045).
+ * stats: nsys entries (host memory). */
+int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys,
+             const mx_options* opt, mx_stats* stats, void* stream);
+
+/* Continuous-batching session: systems are admitted into slots while others iterate;
+ * no counterpart in the reference (its solves are one at a time).  A session owns its
+ * device tables; vectors stay caller-owned. */
+typedef struct mx_session mx_session;
+mx_session* mx_session_create(int32_t capacity, const mx_options* opt);
+/* admit one system (operator copied into the session); returns the slot or < 0 */
+int mx_session_admit(mx_session* s, const mx_operator* op, const mx_system* sys, void* stream);
+/* run `iterations` COCG iterations on every active slot */
+int mx_session_step(mx_session* s, int32_t iterations, void* stream);
+/* host copy of every slot's stats; done[i] = 1 when slot i finished (free to retire) */
+int mx_session_poll(mx_session* s, mx_stats* stats, int32_t* done, void* stream);
+int mx_session_retire(mx_session* s, int32_t slot);
+void mx_session_destroy(mx_session* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEANAS_FDFD_H */

@@ -1,0 +1,380 @@
+// Matrix-free apply of the symmetrised FDFD wave operator (see common.hpp for the
+// operator algebra), with fused epilogues for COCG and FP64 residual replacement.
+//
+// Reference operator: meanas/fdfd/operators.py:40-96 (e_full) scaled by
+// meanas/fdfd/operators.py:99-123 (e_full_preconditioners) as in
+// meanas/fdfd/solvers.py:118-125.  Curl conventions: meanas/fdmath/operators.py:99-171,
+// 222-251 (forward differences on E, backward on H, periodic wrap).
+//
+// Kernel shape: a block owns a TY x 64 tile of (y, z) columns and marches along x over
+// `slab` planes.  Per plane it reads the 3 components of the next x-plane of the unknown
+// (tile + one-cell halo in y and z) once from HBM into a double-buffered LDS plane,
+// computes H on the tile plus a one-cell halo on the minus side into LDS, and forms the
+// output from H(i), its y-/z- neighbours in LDS and H(i-1) kept in registers.  z is the
+// contiguous axis: a wave reads 64 consecutive complex values per row.
+#include "common.hpp"
+
+namespace mx {
+
+enum : int { EPI_PLAIN = 0, EPI_COCG = 1, EPI_RESID = 2 };
+
+__device__ __forceinline__ int wrapi(int v, int n) {
+  int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> load_c(const T* base, int idx) {
+  const cplx<T>* p = reinterpret_cast<const cplx<T>*>(base);
+  return p[idx];
+}
+
+// material value (eps or 1/mu) of component c at cell `cell`
+template <typename T>
+__device__ __forceinline__ cplx<T> load_mat(const T* base, bool is_complex, bool aniso, int c,
+                                            int cell, int ncell) {
+  const int idx = aniso ? c * ncell + cell : cell;
+  if (is_complex) return load_c<T>(base, idx);
+  return cplx<T>{base[idx], T(0)};
+}
+
+template <typename T, int TY, int EPI, typename RT>
+__global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__ ops,
+                                                   const SysDesc* __restrict__ systems,
+                                                   const BlockMap* __restrict__ bmap,
+                                                   int in_slot, int out_slot,
+                                                   int skip_flags, int need_flags,
+                                                   const void* in_ptr, void* out_ptr) {
+  constexpr int TZ = 64;
+  constexpr int LY = TY + 2, LZ = TZ + 2;
+  static_assert(TY >= 2, "tile needs at least two rows");
+  __shared__ cplx<T> sE[2][3][LY][LZ];
+  __shared__ uint8_t sM[2][LY][LZ];
+  __shared__ cplx<T> sH[3][TY + 1][TZ + 1];
+  __shared__ double s_red[4 * 16];
+  __shared__ int s_flag;
+
+  const BlockMap bm = bmap[blockIdx.x];
+  const SysDesc& S = systems[bm.sys];
+  SysState* st = S.st;
+  bool mr = false;   // minimal-residual step: accumulate q^H p and ||q||^2 instead of p^T q
+  if (EPI != EPI_PLAIN) {
+    const int f = st->flags;
+    if ((f & skip_flags) || (need_flags && !(f & need_flags))) return;
+    mr = (f & SF_MR) != 0;
+  }
+  const OpView<T> op = ops[S.op];
+  const int nx = op.nx, ny = op.ny, nz = op.nz;
+  const long long plane = (long long)ny * nz;
+  const long long ncell = plane * nx;
+  const int ntj = (ny + TY - 1) / TY, ntk = (nz + TZ - 1) / TZ;
+  const int tiles = ntj * ntk;
+  const int slab_id = bm.local / tiles, tile = bm.local - slab_id * tiles;
+  const int j0 = (tile / ntk) * TY, k0 = (tile % ntk) * TZ;
+  const int ib = slab_id * S.slab;
+  const int ie = min(ib + S.slab, nx);
+  const int tz = threadIdx.x, ty = threadIdx.y;
+  const int j = j0 + ty, k = k0 + tz;
+  const bool valid = (j < ny) && (k < nz);
+  const int jw = wrapi(j, ny), kw = wrapi(k, nz);
+
+  const bool has_mask = (op.flags & OPF_MASK) != 0;
+  const bool has_mu = (op.flags & OPF_MU) != 0;
+  const AxisCoef<T>* cx = op.coef;
+  const AxisCoef<T>* cy = op.coef + nx;
+  const AxisCoef<T>* cz = op.coef + nx + ny;
+
+  const T* __restrict__ xin = reinterpret_cast<const T*>(in_ptr ? in_ptr : S.vec[in_slot]);
+
+  // ---- load sites of this thread: own column, optional y-halo row, optional z-halo
+  // column.  Indices are 32-bit: the host guarantees 3N < 2^31.
+  const int own_off = jw * nz + kw;
+  const int own_lds = (ty + 1) * LZ + (tz + 1);
+  const bool has_row = (ty == 0) || (ty == TY - 1);
+  const int row_off = wrapi(ty == 0 ? j0 - 1 : j0 + TY, ny) * nz + kw;
+  const int row_lds = (ty == 0 ? 0 : TY + 1) * LZ + (tz + 1);
+  // z-halo columns (k0-1 and k0+TZ) for rows 0..TY+1: lanes of wave TY/2
+  const bool has_col = (ty == TY / 2) && (tz < 2 * LY);
+  const int col_r = tz >> 1, col_side = tz & 1;
+  const int col_off = has_col ? wrapi(j0 - 1 + col_r, ny) * nz + wrapi(col_side ? k0 + TZ : k0 - 1, nz) : 0;
+  const int col_lds = col_r * LZ + (col_side ? TZ + 1 : 0);
+  const int nc = (int)ncell;
+
+  cplx<T> pre_o[3], pre_r[3], pre_c[3];
+  uint8_t m_o = 0, m_r = 0, m_c = 0;
+
+  auto fetch = [&](int ip) {
+    const int pb = ip * (int)plane;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pre_o[c] = load_c<T>(xin, c * nc + pb + own_off);
+    if (has_mask) m_o = op.mask[pb + own_off];
+    if (has_row) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pre_r[c] = load_c<T>(xin, c * nc + pb + row_off);
+      if (has_mask) m_r = op.mask[pb + row_off];
+    }
+    if (has_col) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pre_c[c] = load_c<T>(xin, c * nc + pb + col_off);
+      if (has_mask) m_c = op.mask[pb + col_off];
+    }
+  };
+  auto put = [&](int buf, int lds, const cplx<T> (&v)[3], uint8_t m) {
+    cplx<T>* base = &sE[buf][0][0][0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) base[c * LY * LZ + lds] = (m >> c) & 1 ? zero<T>() : v[c];
+    (&sM[buf][0][0])[lds] = m;
+  };
+  auto commit = [&](int buf) {
+    put(buf, own_lds, pre_o, m_o);
+    if (has_row) put(buf, row_lds, pre_r, m_r);
+    if (has_col) put(buf, col_lds, pre_c, m_c);
+  };
+
+  // per-thread y/z coefficients (wrapped indices)
+  const AxisCoef<T> cyj = cy[jw], czk = cz[kw];
+  const cplx<T> fa_ym = cy[wrapi(j - 1, ny)].fa, fa_zm = cz[wrapi(k - 1, nz)].fa;
+  const AxisCoef<T> cyh = cy[wrapi(j0 - 1, ny)];   // halo row j0-1
+  const AxisCoef<T> czh = cz[wrapi(k0 - 1, nz)];   // halo column k0-1
+
+  // H multiplier at (cell, component): pmc keep * 1/mu
+  auto hscale = [&](cplx<T> h, int c, uint8_t m, int cell) -> cplx<T> {
+    if ((m >> (3 + c)) & 1) return zero<T>();
+    if (has_mu) {
+      const cplx<T> im = load_mat<T>(op.inv_mu, op.flags & OPF_MU_COMPLEX, op.flags & OPF_MU_ANISO, c, cell, nc);
+      h = im * h;
+    }
+    return h;
+  };
+
+  // ---- prologue: H(ib-1) at the own column (y, z components)
+  cplx<T> hpy, hpz;
+  {
+    const int im1 = wrapi(ib - 1, nx);
+    fetch(im1);
+    commit(0);
+    fetch(ib);
+    commit(1);
+    __syncthreads();
+    const AxisCoef<T> cxm = cx[im1];
+    const int r = ty + 1, q = tz + 1;
+    const cplx<T>* A = &sE[0][0][0][0];
+    const cplx<T>* B = &sE[1][0][0][0];
+    auto a_ = [&](int c, int rr, int qq) { return A[(c * LY + rr) * LZ + qq]; };
+    auto b_ = [&](int c, int rr, int qq) { return B[(c * LY + rr) * LZ + qq]; };
+    cplx<T> hy = (czk.fa * a_(0, r, q + 1) - czk.fb * a_(0, r, q)) - (cxm.fa * b_(2, r, q) - cxm.fb * a_(2, r, q));
+    cplx<T> hz = (cxm.fa * b_(1, r, q) - cxm.fb * a_(1, r, q)) - (cyj.fa * a_(0, r + 1, q) - cyj.fb * a_(0, r, q));
+    const uint8_t m = sM[0][r][q];
+    const int cell = im1 * (int)plane + own_off;
+    hpy = hscale(hy, 1, m, cell);
+    hpz = hscale(hz, 2, m, cell);
+    fetch(wrapi(ib + 1, nx));                     // plane ib+1, committed by the first step
+    __syncthreads();
+  }
+
+  double acc[3] = {0.0, 0.0, 0.0};
+  const cplx<T> w2 = op.w2;
+  int cb = 1;
+  T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
+  const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
+  RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
+
+  for (int i = ib; i < ie; ++i) {
+    const int nb = cb ^ 1;
+    commit(nb);                                   // plane i+1
+    if (i + 1 < ie) fetch(wrapi(i + 2, nx));      // prefetch plane i+2
+    const int cell = i * (int)plane + own_off;
+    cplx<T> epsv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      epsv[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
+    const AxisCoef<T> cxi = cx[i];
+    const cplx<T> fa_xm = cx[wrapi(i - 1, nx)].fa;
+    __syncthreads();
+
+    const cplx<T>* C = &sE[cb][0][0][0];
+    const cplx<T>* N = &sE[nb][0][0][0];
+    auto e_ = [&](int c, int rr, int qq) { return C[(c * LY + rr) * LZ + qq]; };
+    auto n_ = [&](int c, int rr, int qq) { return N[(c * LY + rr) * LZ + qq]; };
+    const int r = ty + 1, q = tz + 1;
+    cplx<T> ec[3] = {e_(0, r, q), e_(1, r, q), e_(2, r, q)};
+    const uint8_t mown = sM[cb][r][q];
+    {
+      cplx<T> hx = (cyj.fa * e_(2, r + 1, q) - cyj.fb * ec[2]) - (czk.fa * e_(1, r, q + 1) - czk.fb * ec[1]);
+      cplx<T> hy = (czk.fa * e_(0, r, q + 1) - czk.fb * ec[0]) - (cxi.fa * n_(2, r, q) - cxi.fb * ec[2]);
+      cplx<T> hz = (cxi.fa * n_(1, r, q) - cxi.fb * ec[1]) - (cyj.fa * e_(0, r + 1, q) - cyj.fb * ec[0]);
+      sH[0][r][q] = hscale(hx, 0, mown, cell);
+      sH[1][r][q] = hscale(hy, 1, mown, cell);
+      sH[2][r][q] = hscale(hz, 2, mown, cell);
+    }
+    if (ty == 0) {   // halo row j0-1 (LDS row 0): Hx and Hz are consumed
+      const uint8_t mh = sM[cb][0][q];
+      const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
+      cplx<T> hx = (cyh.fa * e_(2, 1, q) - cyh.fb * e_(2, 0, q)) - (czk.fa * e_(1, 0, q + 1) - czk.fb * e_(1, 0, q));
+      cplx<T> hz = (cxi.fa * n_(1, 0, q) - cxi.fb * e_(1, 0, q)) - (cyh.fa * e_(0, 1, q) - cyh.fb * e_(0, 0, q));
+      sH[0][0][q] = hscale(hx, 0, mh, hcell);
+      sH[2][0][q] = hscale(hz, 2, mh, hcell);
+    }
+    if (ty == TY - 1 && tz < TY) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
+      const int rr = tz + 1;
+      const int jj = wrapi(j0 + tz, ny);
+      const AxisCoef<T> cyr = cy[jj];
+      const uint8_t mh = sM[cb][rr][0];
+      const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
+      cplx<T> hx = (cyr.fa * e_(2, rr + 1, 0) - cyr.fb * e_(2, rr, 0)) - (czh.fa * e_(1, rr, 1) - czh.fb * e_(1, rr, 0));
+      cplx<T> hy = (czh.fa * e_(0, rr, 1) - czh.fb * e_(0, rr, 0)) - (cxi.fa * n_(2, rr, 0) - cxi.fb * e_(2, rr, 0));
+      sH[0][rr][0] = hscale(hx, 0, mh, hcell);
+      sH[1][rr][0] = hscale(hy, 1, mh, hcell);
+    }
+    __syncthreads();
+
+    const cplx<T> hx0 = sH[0][r][q], hy0 = sH[1][r][q], hz0 = sH[2][r][q];
+    const cplx<T> yx = (cyj.fb * hz0 - fa_ym * sH[2][r - 1][q]) - (czk.fb * hy0 - fa_zm * sH[1][r][q - 1]);
+    const cplx<T> yy = (czk.fb * hx0 - fa_zm * sH[0][r][q - 1]) - (cxi.fb * hz0 - fa_xm * hpz);
+    const cplx<T> yz = (cxi.fb * hy0 - fa_xm * hpy) - (cyj.fb * hx0 - fa_ym * sH[0][r - 1][q]);
+    hpy = hy0;
+    hpz = hz0;
+    cplx<T> out[3] = {yx, yy, yz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      out[c] = out[c] - w2 * (epsv[c] * ec[c]);
+      if ((mown >> c) & 1) out[c] = zero<T>();
+    }
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int idx = c * nc + cell;
+        if constexpr (EPI == EPI_RESID) {
+          const cplx<double> bb = load_c<double>(bvec, idx);
+          const cplx<double> rr = bb - cast<double>(out[c]);
+          const cplx<RT> rs = cast<RT>(rr);
+          reinterpret_cast<cplx<RT>*>(rout)[idx] = rs;
+          const cplx<double> rd = cast<double>(rs);
+          acc[0] += norm2(rr);
+          acc[1] += rd.re * rd.re - rd.im * rd.im;
+          acc[2] += 2.0 * rd.re * rd.im;
+        } else {
+          reinterpret_cast<cplx<T>*>(yout)[idx] = out[c];
+          if constexpr (EPI == EPI_COCG) {
+            if (!mr) {
+              const cplx<T> pq = ec[c] * out[c];
+              acc[0] += (double)pq.re;
+              acc[1] += (double)pq.im;
+            } else {
+              const cplx<double> qd = cast<double>(out[c]), pd = cast<double>(ec[c]);
+              acc[0] += qd.re * pd.re + qd.im * pd.im;
+              acc[1] += qd.re * pd.im - qd.im * pd.re;
+              acc[2] += norm2(qd);
+            }
+          }
+        }
+      }
+    }
+    cb = nb;
+  }
+
+  if constexpr (EPI == EPI_PLAIN) return;
+  else {
+    const int nblk = S.nblk_apply;
+    block_sum<3>(acc, s_red);
+    const bool last = publish_partials<3>(acc, S.partial, bm.local, nblk, &st->counter[EPI == EPI_COCG ? 0 : 2], &s_flag);
+    if (!last) return;
+    double tot[3];
+    final_sum<3>(tot, S.partial, nblk, s_red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      st->counter[EPI == EPI_COCG ? 0 : 2] = 0;
+      int f = st->flags;
+      if constexpr (EPI == EPI_COCG) {
+        if (f & SF_MR) {
+          // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
+          // new residual r - alpha A g generic (r^T r no longer ~ 0).  SF_RESTART stays
+          // set: COCG restarts from the new residual.
+          const double qq = tot[2];
+          const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
+          if (!(qq > 0.0) || !finite(alpha.re)) {
+            f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0: no progress possible
+          } else {
+            st->alpha = alpha;
+          }
+        } else {
+          const cplx<double> mu{tot[0], tot[1]};
+          st->mu = mu;
+          const cplx<double> alpha = cdiv(st->rho, mu);
+          if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
+            // p^T A p vanished (or overflowed): replace the residual, then rescue with a
+            // minimal-residual step and restart from p = r
+            f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
+            if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
+          } else {
+            st->alpha = alpha;
+            f &= ~SF_RESTART;
+          }
+        }
+      } else {   // EPI_RESID: residual replacement / initial residual
+        const double rn2 = tot[0];
+        const cplx<double> rho{tot[1], tot[2]};
+        st->true_rnorm2 = rn2;
+        st->rnorm2 = rn2;
+        st->rmax2 = rn2;
+        st->replacements += 1;
+        st->since = 0;
+        // stagnation: replacements that no longer halve the true residual norm
+        if (st->replacements > 1 && rn2 > 0.25 * st->best_true2) st->stalls += 1;
+        else st->stalls = 0;
+        if (st->replacements == 1 || rn2 < st->best_true2) st->best_true2 = rn2;
+        if (!finite(rn2)) f |= SF_NAN | SF_DONE;
+        if (st->stall_limit > 0 && st->stalls >= st->stall_limit) f |= SF_STALL | SF_DONE;
+        if (rn2 <= st->stop2 || (f & (SF_DONE | SF_LIMIT))) {
+          f |= SF_DONE;
+        } else {
+          const cplx<double> ro = st->rho_old;
+          st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
+          st->rho = rho;
+          if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2) f |= SF_RESTART | SF_MR;
+          f &= ~(SF_PAUSE | SF_BREAKDOWN);
+        }
+      }
+      st->flags = f;
+    }
+  }
+}
+
+// ---- host-side launch helpers ------------------------------------------------------
+template <typename T, int EPI, typename RT>
+static hipError_t launch_apply_ty(int ty, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
+                                  int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
+                                  const void* in_ptr = nullptr, void* out_ptr = nullptr) {
+  switch (ty) {
+    case 2: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 4: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 8: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_plain_f32(int ty, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+                                  const void* x, void* y, hipStream_t s) {
+  return launch_apply_ty<float, EPI_PLAIN, float>(ty, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+}
+hipError_t launch_apply_plain_f64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+                                  const void* x, void* y, hipStream_t s) {
+  return launch_apply_ty<double, EPI_PLAIN, double>(ty, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+}
+hipError_t launch_apply_cocg_f32(int ty, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<float, EPI_COCG, float>(ty, ops, sys, bmap, nb, VS_P, VS_Q,
+                                                 SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
+}
+hipError_t launch_apply_cocg_f64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_COCG, double>(ty, ops, sys, bmap, nb, VS_P, VS_Q,
+                                                   SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
+}
+// residual r = b - A x in FP64; r stored as float (mixed mode) or double (FP64 mode)
+hipError_t launch_apply_resid_r32(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, float>(ty, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+}
+hipError_t launch_apply_resid_r64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, double>(ty, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+}
+
+}  // namespace mx

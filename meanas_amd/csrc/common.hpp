@@ -1,0 +1,210 @@
+// Shared device-side types for the FDFD solve library (gfx950).
+//
+// Vector layout (all field vectors): C order (3, nx, ny, nz), z fastest, complex
+// interleaved (re, im) -- the layout of meanas.fdmath.vec (fdmath/vectorization.py:51-68).
+//
+// The operator applied is the symmetrised wave operator of meanas.fdfd.solvers.generic
+// (fdfd/solvers.py:118-125):  A = Pl A0 Pr  with A0 = e_full (fdfd/operators.py:40-96)
+// and Pl = diag(p), Pr = diag(1/p) (fdfd/operators.py:99-123).  With p factored per
+// axis, p_x = se_x[i] sh_y[j] sh_z[k] (se = sqrt(dx_e), sh = sqrt(dx_h); p_y, p_z
+// likewise), A takes the manifestly symmetric form
+//     A = pe ( C^T M C - w^2 eps ) pe ,   M = pm diag(1/mu)
+// where C is the forward curl built from the 1D scaled differences
+//     (D_a f)[n] = fa_a[n] f[n+1] - fb_a[n] f[n],
+//     fa_a[n] = 1 / (se_a[n] sh_a[n+1 mod N]),  fb_a[n] = 1 / (se_a[n] sh_a[n]),
+// and C^T uses  (-D_a^T g)[n] = fb_a[n] g[n] - fa_a[n-1] g[n-1].  So the stencil reads
+// the unknown directly; no per-cell scaling is needed.  p differs from the reference's
+// sqrt(p^2) by at most a per-cell sign, a +-1 similarity that leaves E unchanged.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mx {
+
+template <typename T> struct cplx { T re, im; };
+
+template <typename T> __host__ __device__ __forceinline__ cplx<T> mk(T r, T i) { return {r, i}; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator+(cplx<T> a, cplx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator-(cplx<T> a, cplx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T> __host__ __device__ __forceinline__ cplx<T> operator*(T s, cplx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> conj(cplx<T> a) { return {a.re, -a.im}; }
+template <typename T> __host__ __device__ __forceinline__ T norm2(cplx<T> a) { return a.re * a.re + a.im * a.im; }
+template <typename T> __host__ __device__ __forceinline__ cplx<T> zero() { return {T(0), T(0)}; }
+template <typename D, typename S> __host__ __device__ __forceinline__ cplx<D> cast(cplx<S> a) { return {D(a.re), D(a.im)}; }
+// a*b - c*d  (complex), the shape of every scaled difference
+template <typename T> __device__ __forceinline__ cplx<T> diff2(cplx<T> a, cplx<T> x1, cplx<T> b, cplx<T> x0) {
+  return a * x1 - b * x0;
+}
+__host__ __device__ __forceinline__ cplx<double> cdiv(cplx<double> a, cplx<double> b) {
+  double d = b.re * b.re + b.im * b.im;
+  return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+// Per-axis coefficients at index n: fa multiplies f[n+1], fb multiplies f[n] in the
+// scaled forward difference; se = sqrt(dx_e[n]), sh = sqrt(dx_h[n]) give the diagonal
+// scaling p used to map J -> b and x -> E.
+template <typename T> struct AxisCoef { cplx<T> fa, fb, se, sh; };
+
+// Material / mask format flags of an operator.
+enum : int {
+  OPF_EPS_COMPLEX = 1,   // eps interleaved complex, else real
+  OPF_EPS_ANISO   = 2,   // eps has 3N values (per component), else N (isotropic)
+  OPF_MU          = 4,   // inv_mu present
+  OPF_MU_COMPLEX  = 8,
+  OPF_MU_ANISO    = 16,
+  OPF_MASK        = 32,  // uint8 per cell: bits 0-2 PEC x,y,z ; bits 3-5 PMC x,y,z
+};
+
+// Device view of one operator (one structure) in precision T.
+template <typename T> struct OpView {
+  int nx, ny, nz, flags;
+  cplx<T> w2;                       // omega^2
+  const AxisCoef<T>* coef;          // nx + ny + nz entries: x axis, then y, then z
+  const T* eps;                     // real or interleaved complex, N or 3N entries
+  const T* inv_mu;                  // same conventions, or null
+  const uint8_t* mask;              // N bytes or null
+};
+
+// Solver state flags (per linear system).
+enum : int {
+  SF_PAUSE     = 1,    // kernels skip this system until the host clears it
+  SF_BREAKDOWN = 2,    // p^T A p vanished (COCG breakdown)
+  SF_DONE      = 4,    // converged / finished: kernels skip permanently
+  SF_NAN       = 8,    // non-finite scalar seen
+  SF_RESTART   = 16,   // next p-update uses beta = 0 (p = r)
+  SF_SELECT    = 32,   // selected for a host-driven step (residual / accumulate)
+  SF_INACTIVE  = 64,   // slot not in use
+  SF_LIMIT     = 128,  // iteration limit reached: the next replacement finishes the system
+  SF_MR        = 256,  // next step is a rescue step along a hashed direction (breakdown)
+  SF_STALL     = 512,  // stopped: replacements no longer reduce the true residual
+};
+
+// |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown
+constexpr double kRhoBreakdown = 1e-10;
+
+// Scalars of one linear system, device resident.  Written by the last block of each
+// reduction kernel (deterministic fixed-order sums), read by the following kernels.
+struct SysState {
+  cplx<double> rho;        // r^T r (unconjugated) of the current residual
+  cplx<double> rho_old;    // previous rho (for beta after a residual replacement)
+  cplx<double> alpha, beta;
+  cplx<double> mu;         // p^T A p
+  double rnorm2;           // ||r||^2 of the (recursive) residual
+  double rmax2;            // max recursive ||r||^2 since the last residual replacement
+  double true_rnorm2;      // ||b - A x||^2 in FP64 at the last replacement
+  double bnorm2;           // ||b||^2
+  double stop2;            // recursive-residual target (absolute, squared)
+  double delta2;           // reliable-update trigger: pause when rnorm2 < delta2 * rmax2
+  double rtol, atol;       // convergence: ||b - A x|| <= max(rtol ||b||, atol)
+  double best_true2;       // smallest true ||r||^2 seen at a replacement
+  int flags;
+  int iters;               // inner iterations performed
+  int replacements;        // residual replacements (reliable updates / refinements)
+  int maxiter;
+  int since;               // iterations since the last replacement
+  int stall_iters;         // force a replacement after this many iterations without one
+  int stalls;              // consecutive replacements without a 2x true-residual gain
+  int stall_limit;         // stop (SF_STALL) after this many
+  unsigned int counter[4]; // last-block arrival counters (apply, update, residual, norm)
+};
+
+// One linear system of a batch.  Vector slots name the vectors a kernel reads/writes.
+enum : int { VS_X = 0, VS_R, VS_P, VS_Q, VS_XLO, VS_B, VS_T, VS_COUNT };
+
+struct SysDesc {
+  int op;                     // operator index
+  int nblk_apply;             // blocks of the apply grid owned by this system
+  int nblk_vec;               // blocks of the vector grid owned by this system
+  int slab;                   // x-planes per apply block
+  long long n3;               // 3 * nx * ny * nz
+  void* vec[VS_COUNT];        // device vectors (precision depends on the slot and mode)
+  SysState* st;
+  double* partial;            // >= 4 * max(nblk_apply, nblk_vec) doubles
+};
+
+// Work assignment of one block: system index and block index within that system.
+struct BlockMap { int sys, local; };
+
+// ---- reductions ----------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum K doubles over the block; result valid in thread 0.  `scratch` >= K * nwaves.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* scratch) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthreads = blockDim.x * blockDim.y;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = (nthreads + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) scratch[q * 16 + wave] = v[q];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < nwaves; ++w) s += scratch[q * 16 + w];
+      v[q] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Publish this block's K partial sums; returns true in every thread of the block that
+// arrived last for this system.  That block then owns the fixed-order final sum.
+template <int K>
+__device__ __forceinline__ bool publish_partials(const double (&v)[K], double* partial, int local,
+                                                 int nblk, unsigned int* counter, int* s_flag) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) partial[q * nblk + local] = v[q];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *s_flag = (prev == (unsigned int)(nblk - 1));
+  }
+  __syncthreads();
+  bool last = *s_flag != 0;
+  if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return last;
+}
+
+// In the last block: fixed-order sum of the K partial arrays; result in thread 0.
+template <int K>
+__device__ __forceinline__ void final_sum(double (&out)[K], const double* partial, int nblk, double* scratch) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthreads = blockDim.x * blockDim.y;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double s = 0.0;
+    for (int b = tid; b < nblk; b += nthreads) s += partial[q * nblk + b];
+    out[q] = s;
+  }
+  block_sum<K>(out, scratch);
+}
+
+__device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }
+
+// Deterministic pseudo-random complex direction (entries in [-1/2, 1/2)^2) of vector
+// index e: a 64-bit mix (splitmix64 finaliser) of the index.  Used to restart COCG
+// with a generic residual after a breakdown.
+__device__ __forceinline__ cplx<double> hashed_direction(long long e) {
+  unsigned long long z = (unsigned long long)e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const double u = (double)(z >> 40) * (1.0 / 16777216.0) - 0.5;
+  const double v = (double)((z >> 8) & 0xFFFFFFull) * (1.0 / 16777216.0) - 0.5;
+  return cplx<double>{u, v};
+}
+
+}  // namespace mx

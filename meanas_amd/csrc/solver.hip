@@ -1,0 +1,845 @@
+// Host runtime of the FDFD solve library: C ABI (include/meanas_fdfd.h), work-table
+// construction, hipGraph-captured COCG iteration chunks and the pipelined host loop.
+//
+// Replaces meanas/fdfd/solvers.py:62 (scipy qmr on the assembled CSR matrix) with a
+// batched, matrix-free, complex-symmetric Krylov solve on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "../../include/meanas_fdfd.h"
+
+namespace mx {
+// apply.hip
+hipError_t launch_apply_plain_f32(int ty, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_plain_f64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_cocg_f32(int ty, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_cocg_f64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r32(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+// vector.hip
+hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_update_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_update_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_accum_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_accum_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_bnorm(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_zero_slot(int prec64, const SysDesc*, const BlockMap*, int, int, hipStream_t);
+hipError_t launch_diag(const AxisCoef<double>*, int, int, int, const void*, void*, double, double, int, hipStream_t);
+}  // namespace mx
+
+using namespace mx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+
+#define MX_TRY(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kVecBlock = 256;
+constexpr int kVecElemsPerThread = 8;
+constexpr int kVecMaxBlocks = 2048;
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+long long ncell_of(const mx_operator& op) { return (long long)op.nx * op.ny * op.nz; }
+
+bool valid_op(const mx_operator& op, int mode, std::string* why) {
+  if (op.nx < 1 || op.ny < 1 || op.nz < 1) { *why = "grid dimensions must be >= 1"; return false; }
+  if (ncell_of(op) > (1LL << 31) / 3) { *why = "grid too large (3N must fit in int32 indexing of a plane)"; return false; }
+  const bool need32 = (mode == MX_MIXED);
+  if (!op.coef64 || !op.eps64) { *why = "coef64/eps64 are required"; return false; }
+  if (need32 && (!op.coef32 || !op.eps32)) { *why = "coef32/eps32 are required for the mixed mode"; return false; }
+  if ((op.flags & MX_MU) && (!op.inv_mu64 || (need32 && !op.inv_mu32))) { *why = "MX_MU set but inv_mu is NULL"; return false; }
+  if ((op.flags & MX_MASK) && !op.mask) { *why = "MX_MASK set but mask is NULL"; return false; }
+  return true;
+}
+
+template <typename T>
+OpView<T> view_of(const mx_operator& op) {
+  OpView<T> v;
+  v.nx = op.nx; v.ny = op.ny; v.nz = op.nz; v.flags = op.flags;
+  v.w2 = cplx<T>{(T)op.omega2_re, (T)op.omega2_im};
+  if constexpr (sizeof(T) == 4) {
+    v.coef = reinterpret_cast<const AxisCoef<T>*>(op.coef32);
+    v.eps = reinterpret_cast<const T*>(op.eps32);
+    v.inv_mu = reinterpret_cast<const T*>(op.inv_mu32);
+  } else {
+    v.coef = reinterpret_cast<const AxisCoef<T>*>(op.coef64);
+    v.eps = reinterpret_cast<const T*>(op.eps64);
+    v.inv_mu = reinterpret_cast<const T*>(op.inv_mu64);
+  }
+  v.mask = op.mask;
+  return v;
+}
+
+int auto_tile_rows(const mx_operator& op) { return op.ny >= 8 ? 4 : 2; }
+
+int auto_slab(const mx_operator& op, int ty) {
+  // enough blocks to fill 256 CUs several times over, but deep slabs to amortise the
+  // two extra planes a slab reads
+  const long long tiles = (long long)((op.ny + ty - 1) / ty) * ((op.nz + 63) / 64);
+  int slab = 16;
+  while (slab > 4 && tiles * ((op.nx + slab - 1) / slab) < 2048) slab /= 2;
+  return std::min(slab, op.nx);
+}
+
+int apply_blocks(const mx_operator& op, int ty, int slab) {
+  const long long tiles = (long long)((op.ny + ty - 1) / ty) * ((op.nz + 63) / 64);
+  return (int)(tiles * ((op.nx + slab - 1) / slab));
+}
+
+int apply_blocks_upper(const mx_operator& op) {   // ty = 2, slab = 1
+  return (int)((long long)((op.ny + 1) / 2) * ((op.nz + 63) / 64) * op.nx);
+}
+
+int vec_blocks(long long n3) {
+  long long nb = (n3 + (long long)kVecBlock * kVecElemsPerThread - 1) / ((long long)kVecBlock * kVecElemsPerThread);
+  return (int)std::max(1LL, std::min<long long>(nb, kVecMaxBlocks));
+}
+
+size_t elem_bytes(int mode) { return mode == MX_MIXED ? 8 : 16; }
+
+// workspace layout per system: r, p, q, x_lo (inner precision), state, partials
+struct WorkLayout {
+  size_t off_r, off_p, off_q, off_xlo, off_state, off_partial, total;
+};
+
+WorkLayout work_layout(const mx_operator& op, int mode) {
+  const long long n3 = 3 * ncell_of(op);
+  const size_t vb = align_up((size_t)n3 * elem_bytes(mode));
+  WorkLayout w;
+  w.off_r = 0;
+  w.off_p = w.off_r + vb;
+  w.off_q = w.off_p + vb;
+  w.off_xlo = w.off_q + vb;
+  w.off_state = w.off_xlo + vb;
+  w.off_partial = w.off_state + align_up(sizeof(SysState));
+  const size_t nbmax = (size_t)std::max(apply_blocks_upper(op), kVecMaxBlocks);
+  w.total = w.off_partial + align_up(4 * nbmax * sizeof(double));
+  return w;
+}
+
+// Order the blocks of a grid so that each XCD (blocks are dispatched round-robin over
+// 8 XCDs) receives a contiguous run of neighbouring tiles: better L2 reuse of halos and
+// of shared materials.  `natural` is the work list in spatial order.
+std::vector<BlockMap> xcd_order(const std::vector<BlockMap>& natural) {
+  const size_t n = natural.size();
+  constexpr size_t kXcd = 8;
+  std::vector<BlockMap> out(n);
+  const size_t per = n / kXcd, rem = n % kXcd;
+  // blockIdx w -> XCD (w % 8), position (w / 8) within that XCD's run
+  std::vector<size_t> start(kXcd + 1, 0);
+  for (size_t x = 0; x < kXcd; ++x) start[x + 1] = start[x] + per + (x < rem ? 1 : 0);
+  for (size_t w = 0; w < n; ++w) {
+    const size_t x = w % kXcd, pos = w / kXcd;
+    out[w] = natural[start[x] + pos];
+  }
+  return out;
+}
+
+// ---- a batch of systems with device-resident tables ---------------------------------
+struct Batch {
+  int mode = MX_MIXED;
+  int ty = 4;
+  int nops = 0, nsys = 0;
+  int nb_apply = 0, nb_vec = 0;
+  char* d_mem = nullptr;
+  OpView<float>* d_ops32 = nullptr;
+  OpView<double>* d_ops64 = nullptr;
+  SysDesc* d_sys = nullptr;
+  BlockMap* d_bmap_apply = nullptr;
+  BlockMap* d_bmap_vec = nullptr;
+  std::vector<SysDesc> h_sys;
+  SysState* h_state[2] = {nullptr, nullptr};   // pinned, double-buffered polls
+  hipStream_t stream = nullptr;                  // library-owned capture stream
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_chunk = 0;
+
+  ~Batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    for (auto& h : h_state) if (h) (void)hipHostFree(h);
+    if (d_mem) (void)hipFree(d_mem);
+  }
+};
+
+int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys, int nsys, const mx_options& o) {
+  std::string why;
+  B.mode = o.mode;
+  B.nops = nops;
+  B.nsys = nsys;
+  int ty = o.tile_rows;
+  if (ty == 0) {
+    int mny = 1 << 30;
+    for (int i = 0; i < nops; ++i) mny = std::min(mny, ops[i].ny);
+    ty = auto_tile_rows(ops[0]);
+    if (mny < 8) ty = 2;
+  }
+  if (ty != 2 && ty != 4 && ty != 8) return fail("tile_rows must be 2, 4 or 8");
+  B.ty = ty;
+  for (int i = 0; i < nops; ++i)
+    if (!valid_op(ops[i], o.mode, &why)) return fail("operator " + std::to_string(i) + ": " + why);
+
+  std::vector<BlockMap> nat_apply, nat_vec;
+  B.h_sys.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    const mx_system& y = sys[s];
+    if (y.op < 0 || y.op >= nops) return fail("system " + std::to_string(s) + ": bad operator index");
+    if (!y.b || !y.x || !y.work) return fail("system " + std::to_string(s) + ": b, x and work are required");
+    const mx_operator& op = ops[y.op];
+    const WorkLayout w = work_layout(op, o.mode);
+    char* wk = reinterpret_cast<char*>(y.work);
+    SysDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.op = y.op;
+    d.slab = o.slab > 0 ? std::min(o.slab, op.nx) : auto_slab(op, ty);
+    d.nblk_apply = apply_blocks(op, ty, d.slab);
+    d.n3 = 3 * ncell_of(op);
+    d.nblk_vec = vec_blocks(d.n3);
+    d.vec[VS_X] = y.x;
+    d.vec[VS_B] = const_cast<void*>(y.b);
+    d.vec[VS_R] = wk + w.off_r;
+    d.vec[VS_P] = wk + w.off_p;
+    d.vec[VS_Q] = wk + w.off_q;
+    d.vec[VS_XLO] = wk + w.off_xlo;
+    d.st = reinterpret_cast<SysState*>(wk + w.off_state);
+    d.partial = reinterpret_cast<double*>(wk + w.off_partial);
+    B.h_sys[s] = d;
+    for (int b = 0; b < d.nblk_apply; ++b) nat_apply.push_back({s, b});
+    for (int b = 0; b < d.nblk_vec; ++b) nat_vec.push_back({s, b});
+  }
+  const std::vector<BlockMap> bm_apply = xcd_order(nat_apply);
+  const std::vector<BlockMap> bm_vec = xcd_order(nat_vec);
+  B.nb_apply = (int)bm_apply.size();
+  B.nb_vec = (int)bm_vec.size();
+
+  const size_t s_ops32 = align_up(sizeof(OpView<float>) * nops);
+  const size_t s_ops64 = align_up(sizeof(OpView<double>) * nops);
+  const size_t s_sys = align_up(sizeof(SysDesc) * nsys);
+  const size_t s_ba = align_up(sizeof(BlockMap) * bm_apply.size());
+  const size_t s_bv = align_up(sizeof(BlockMap) * bm_vec.size());
+  MX_TRY(hipMalloc(&B.d_mem, s_ops32 + s_ops64 + s_sys + s_ba + s_bv));
+  char* p = B.d_mem;
+  B.d_ops32 = reinterpret_cast<OpView<float>*>(p); p += s_ops32;
+  B.d_ops64 = reinterpret_cast<OpView<double>*>(p); p += s_ops64;
+  B.d_sys = reinterpret_cast<SysDesc*>(p); p += s_sys;
+  B.d_bmap_apply = reinterpret_cast<BlockMap*>(p); p += s_ba;
+  B.d_bmap_vec = reinterpret_cast<BlockMap*>(p);
+
+  std::vector<OpView<float>> v32(nops);
+  std::vector<OpView<double>> v64(nops);
+  for (int i = 0; i < nops; ++i) {
+    v64[i] = view_of<double>(ops[i]);
+    if (o.mode == MX_MIXED) v32[i] = view_of<float>(ops[i]);
+    else std::memset(&v32[i], 0, sizeof(v32[i]));
+  }
+  MX_TRY(hipMemcpy(B.d_ops32, v32.data(), sizeof(OpView<float>) * nops, hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(B.d_ops64, v64.data(), sizeof(OpView<double>) * nops, hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(B.d_sys, B.h_sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(B.d_bmap_apply, bm_apply.data(), sizeof(BlockMap) * bm_apply.size(), hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(B.d_bmap_vec, bm_vec.data(), sizeof(BlockMap) * bm_vec.size(), hipMemcpyHostToDevice));
+
+  for (auto& h : B.h_state) MX_TRY(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(SysState) * nsys, hipHostMallocDefault));
+  MX_TRY(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+  for (auto& e : B.ev) MX_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return 0;
+}
+
+// one COCG iteration over every runnable system of the batch
+int enqueue_iteration(Batch& B, hipStream_t s) {
+  const bool mixed = B.mode == MX_MIXED;
+  if (mixed) {
+    MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+    MX_TRY(launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  } else {
+    MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+    MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  }
+  return 0;
+}
+
+// residual replacement on paused systems: x += x_lo ; r = b - A x (FP64)
+int enqueue_replacement(Batch& B, hipStream_t s) {
+  if (B.mode == MX_MIXED) {
+    MX_TRY(launch_accum_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+    MX_TRY(launch_apply_resid_r32(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+  } else {
+    MX_TRY(launch_accum_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+    MX_TRY(launch_apply_resid_r64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+  }
+  return 0;
+}
+
+int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
+  for (int it = 0; it < iters; ++it) {
+    if (enqueue_iteration(B, s)) return -1;
+    if ((it + 1) % check_every == 0 || it + 1 == iters)
+      if (enqueue_replacement(B, s)) return -1;
+  }
+  return 0;
+}
+
+void fill_stats(const SysState& st, mx_stats* out) {
+  out->iterations = st.iters;
+  out->replacements = st.replacements;
+  out->flags = st.flags;
+  out->bnorm = std::sqrt(st.bnorm2);
+  out->rel_residual = st.bnorm2 > 0 ? std::sqrt(st.true_rnorm2 / st.bnorm2) : std::sqrt(st.true_rnorm2);
+  if (st.flags & SF_NAN) out->status = MX_NONFINITE;
+  else if (st.true_rnorm2 <= st.stop2) out->status = MX_CONVERGED;
+  else if (st.flags & SF_STALL) out->status = MX_STALLED;
+  else if (st.flags & SF_BREAKDOWN) out->status = MX_BREAKDOWN;
+  else out->status = MX_MAXITER;
+}
+
+mx_options defaults(const mx_options* opt) {
+  mx_options o;
+  std::memset(&o, 0, sizeof(o));
+  if (opt) o = *opt;
+  if (o.chunk <= 0) o.chunk = 16;
+  if (o.check_every <= 0) o.check_every = (o.mode == MX_MIXED) ? 1 : o.chunk;
+  if (o.delta <= 0) o.delta = (o.mode == MX_MIXED) ? 0.1 : 0.0;
+  if (o.stall_iters == 0) o.stall_iters = 1000;
+  if (o.stall_limit == 0) o.stall_limit = 4;
+  return o;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+int mx_abi_version(void) { return MX_ABI_VERSION; }
+
+const char* mx_last_error(void) { return g_err.c_str(); }
+
+size_t mx_work_bytes(const mx_operator* op, int32_t mode) {
+  if (!op || (mode != MX_MIXED && mode != MX_FP64)) return 0;
+  return work_layout(*op, mode).total;
+}
+
+// ---- apply plans -----------------------------------------------------------------
+struct mx_plan {
+  int mode, ty, nb;
+  char* d_mem;
+};
+
+mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, int32_t slab) {
+  std::string why;
+  if (!op) { fail("null operator"); return nullptr; }
+  if (mode != MX_MIXED && mode != MX_FP64) { fail("mode must be MX_MIXED or MX_FP64"); return nullptr; }
+  if (!valid_op(*op, mode, &why)) { fail(why); return nullptr; }
+  const int ty = tile_rows ? tile_rows : (op->ny >= 8 ? 4 : 2);
+  if (ty != 2 && ty != 4 && ty != 8) { fail("tile_rows must be 2, 4 or 8"); return nullptr; }
+  SysDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.op = 0;
+  d.slab = slab > 0 ? std::min<int>(slab, op->nx) : auto_slab(*op, ty);
+  d.nblk_apply = apply_blocks(*op, ty, d.slab);
+  d.n3 = 3 * ncell_of(*op);
+  std::vector<BlockMap> nat;
+  for (int b = 0; b < d.nblk_apply; ++b) nat.push_back({0, b});
+  std::vector<BlockMap> bm = xcd_order(nat);
+  const size_t s_op = align_up(std::max(sizeof(OpView<float>), sizeof(OpView<double>)));
+  const size_t s_sys = align_up(sizeof(SysDesc));
+  const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
+  mx_plan* P = new (std::nothrow) mx_plan{mode, ty, d.nblk_apply, nullptr};
+  if (!P) { fail("out of host memory"); return nullptr; }
+  hipError_t e = hipMalloc(&P->d_mem, s_op + s_sys + s_bm);
+  if (e != hipSuccess) { fail(std::string("hipMalloc: ") + hipGetErrorString(e)); delete P; return nullptr; }
+  if (mode == MX_MIXED) {
+    OpView<float> v = view_of<float>(*op);
+    e = hipMemcpy(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
+  } else {
+    OpView<double> v = view_of<double>(*op);
+    e = hipMemcpy(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipMemcpy(P->d_mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(P->d_mem + s_op + s_sys, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { fail(std::string("hipMemcpy: ") + hipGetErrorString(e)); (void)hipFree(P->d_mem); delete P; return nullptr; }
+  return P;
+}
+
+int mx_plan_apply(mx_plan* P, const void* x, void* y, void* stream) {
+  if (!P || !x || !y) return fail("null argument");
+  const size_t s_op = align_up(std::max(sizeof(OpView<float>), sizeof(OpView<double>)));
+  const size_t s_sys = align_up(sizeof(SysDesc));
+  const SysDesc* sys = reinterpret_cast<const SysDesc*>(P->d_mem + s_op);
+  const BlockMap* bm = reinterpret_cast<const BlockMap*>(P->d_mem + s_op + s_sys);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (P->mode == MX_MIXED)
+    MX_TRY(launch_apply_plain_f32(P->ty, reinterpret_cast<const OpView<float>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+  else
+    MX_TRY(launch_apply_plain_f64(P->ty, reinterpret_cast<const OpView<double>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+  return 0;
+}
+
+int mx_plan_blocks(const mx_plan* P) { return P ? P->nb : -1; }
+
+void mx_plan_destroy(mx_plan* P) {
+  if (!P) return;
+  if (P->d_mem) (void)hipFree(P->d_mem);
+  delete P;
+}
+
+int mx_apply(const mx_operator* op, int32_t mode, const void* x, void* y, void* stream) {
+  mx_plan* P = mx_plan_create(op, mode, 0, 0);
+  if (!P) return -1;
+  int rc = mx_plan_apply(P, x, y, stream);
+  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+  mx_plan_destroy(P);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const void* in, void* out, void* stream) {
+  if (!op || !in || !out || !op->coef64) return fail("null argument");
+  if (map < 0 || map > 3) return fail("unknown map");
+  MX_TRY(launch_diag(reinterpret_cast<const AxisCoef<double>*>(op->coef64), op->nx, op->ny, op->nz, in, out, c_re, c_im,
+                     map, reinterpret_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream) {
+  if (!op || !b || !x || !r) return fail("null argument");
+  std::string why;
+  if (!valid_op(*op, MX_FP64, &why)) return fail(why);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int ty = op->ny >= 8 ? 4 : 2;
+  SysDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.slab = auto_slab(*op, ty);
+  d.nblk_apply = apply_blocks(*op, ty, d.slab);
+  d.n3 = 3 * ncell_of(*op);
+  d.nblk_vec = vec_blocks(d.n3);
+  d.vec[VS_X] = const_cast<void*>(x);
+  d.vec[VS_B] = const_cast<void*>(b);
+  d.vec[VS_R] = r;
+  std::vector<BlockMap> bm;
+  for (int i = 0; i < d.nblk_apply; ++i) bm.push_back({0, i});
+  const size_t s_op = align_up(sizeof(OpView<double>));
+  const size_t s_sys = align_up(sizeof(SysDesc));
+  const size_t s_st = align_up(sizeof(SysState));
+  const size_t s_part = align_up(sizeof(double) * 4 * (size_t)d.nblk_apply);
+  const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
+  char* mem = nullptr;
+  MX_TRY(hipMalloc(&mem, s_op + s_sys + s_st + s_part + s_bm));
+  d.st = reinterpret_cast<SysState*>(mem + s_op + s_sys);
+  d.partial = reinterpret_cast<double*>(mem + s_op + s_sys + s_st);
+  SysState st;
+  std::memset(&st, 0, sizeof(st));
+  st.flags = SF_PAUSE;
+  st.stop2 = -1.0;
+  OpView<double> v = view_of<double>(*op);
+  int rc = 0;
+  hipError_t e = hipMemcpy(mem, &v, sizeof(v), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d.st, &st, sizeof(st), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(mem + s_op + s_sys + s_st + s_part, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_apply_resid_r64(ty, reinterpret_cast<const OpView<double>*>(mem), reinterpret_cast<const SysDesc*>(mem + s_op),
+                                                 reinterpret_cast<const BlockMap*>(mem + s_op + s_sys + s_st + s_part), d.nblk_apply, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) rc = fail(std::string("mx_residual: ") + hipGetErrorString(e));
+  else if (norm2_out) *norm2_out = st.true_rnorm2;
+  (void)hipFree(mem);
+  return rc;
+}
+
+int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys, const mx_options* opt,
+             mx_stats* stats, void* stream) {
+  if (!ops || nops < 1 || !systems || nsys < 1) return fail("mx_solve: empty batch");
+  const mx_options o = defaults(opt);
+  if (o.mode != MX_MIXED && o.mode != MX_FP64) return fail("mode must be MX_MIXED or MX_FP64");
+  Batch B;
+  if (build_batch(B, ops, nops, systems, nsys, o)) return -1;
+  hipStream_t user = reinterpret_cast<hipStream_t>(stream);
+  hipStream_t s = B.stream;
+  // order after the caller's pending work (inputs are produced on the caller's stream)
+  MX_TRY(hipEventRecord(B.ev[0], user));
+  MX_TRY(hipStreamWaitEvent(s, B.ev[0], 0));
+
+  // initial state: paused + restart, so the first replacement computes r = b - A x0
+  std::vector<SysState> init(nsys);
+  int maxiter_all = 0;
+  for (int i = 0; i < nsys; ++i) {
+    SysState& st = init[i];
+    std::memset(&st, 0, sizeof(st));
+    st.flags = SF_PAUSE | SF_RESTART;
+    st.rtol = systems[i].rtol;
+    st.atol = systems[i].atol;
+    st.maxiter = std::max(1, systems[i].maxiter);
+    st.delta2 = o.delta * o.delta;
+    st.stall_iters = std::max(0, o.stall_iters);
+    st.stall_limit = std::max(0, o.stall_limit);
+    maxiter_all = std::max(maxiter_all, st.maxiter);
+  }
+  for (int i = 0; i < nsys; ++i) MX_TRY(hipMemcpyAsync(B.h_sys[i].st, &init[i], sizeof(SysState), hipMemcpyHostToDevice, s));
+  if (launch_zero_slot(B.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) return fail("zero x_lo");
+  MX_TRY(launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  if (enqueue_replacement(B, s)) return -1;
+
+  const int chunk = o.chunk;
+  const int check = std::min(o.check_every, chunk);
+  if (o.use_graph) {
+    MX_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_chunk(B, s, chunk, check);
+    hipError_t ce = hipStreamEndCapture(s, &B.graph);
+    if (rc) return -1;
+    MX_TRY(ce);
+    MX_TRY(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
+  }
+  auto launch_chunk = [&]() -> int {
+    if (o.use_graph) {
+      MX_TRY(hipGraphLaunch(B.gexec, s));
+      return 0;
+    }
+    return enqueue_chunk(B, s, chunk, check);
+  };
+  auto poll = [&](int slot) -> int {
+    for (int i = 0; i < nsys; ++i)
+      MX_TRY(hipMemcpyAsync(&B.h_state[slot][i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost, s));
+    MX_TRY(hipEventRecord(B.ev[slot], s));
+    return 0;
+  };
+  auto all_done = [&](int slot) {
+    for (int i = 0; i < nsys; ++i)
+      if (!(B.h_state[slot][i].flags & SF_DONE)) return false;
+    return true;
+  };
+  // pipelined: keep one chunk queued behind the one whose state is being inspected
+  const long long max_chunks = (long long)maxiter_all / chunk + 4;
+  if (poll(0)) return -1;
+  if (launch_chunk() || poll(1)) return -1;
+  int cur = 0;
+  for (long long c = 0;; ++c) {
+    MX_TRY(hipEventSynchronize(B.ev[cur]));
+    if (all_done(cur)) break;
+    if (c > max_chunks) return fail("mx_solve: iteration limit exceeded without completion (internal error)");
+    if (launch_chunk() || poll(cur)) return -1;
+    cur ^= 1;
+  }
+  MX_TRY(hipStreamSynchronize(s));
+  // final states (the queued chunk, if any, was a no-op on finished systems)
+  std::vector<SysState> fin(nsys);
+  for (int i = 0; i < nsys; ++i) MX_TRY(hipMemcpy(&fin[i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
+  if (stats)
+    for (int i = 0; i < nsys; ++i) fill_stats(fin[i], &stats[i]);
+  MX_TRY(hipEventRecord(B.ev[0], s));
+  MX_TRY(hipStreamWaitEvent(user, B.ev[0], 0));
+  return 0;
+}
+
+
+// ---- continuous-batching session --------------------------------------------------
+// Slots hold admitted systems.  Device tables (operators, system descriptors) have one
+// entry per slot; the block maps of every step cover the active slots only and are
+// rebuilt when the active set changes.  Iteration chunks are captured as hipGraphs and
+// cached per active-set version.
+struct mx_session {
+  mx_options o;
+  int capacity = 0;
+  char* d_tab = nullptr;                 // ops32[cap], ops64[cap], sys[cap]
+  OpView<float>* d_ops32 = nullptr;
+  OpView<double>* d_ops64 = nullptr;
+  SysDesc* d_sys = nullptr;
+  BlockMap* d_bm = nullptr;              // combined maps: apply blocks, then vector blocks
+  size_t bm_cap = 0;
+  BlockMap* d_adm = nullptr;             // maps of the slot being admitted
+  size_t adm_cap = 0;
+  std::vector<SysDesc> h_sys;
+  std::vector<mx_operator> h_ops;
+  std::vector<char> active;
+  std::vector<int> ty_of;                // tile rows used by each slot (all equal here)
+  int ty = 4;
+  int nb_apply = 0, nb_vec = 0;
+  bool dirty = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  SysState* h_state = nullptr;           // pinned, capacity entries
+  BlockMap* h_bm = nullptr;              // pinned staging for maps
+  size_t h_bm_cap = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_iters = 0;
+
+  ~mx_session() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (h_state) (void)hipHostFree(h_state);
+    if (h_bm) (void)hipHostFree(h_bm);
+    if (d_bm) (void)hipFree(d_bm);
+    if (d_adm) (void)hipFree(d_adm);
+    if (d_tab) (void)hipFree(d_tab);
+  }
+};
+
+namespace {
+
+int ensure_device(BlockMap** d, size_t* cap, size_t need, hipStream_t s) {
+  if (need <= *cap) return 0;
+  MX_TRY(hipStreamSynchronize(s));
+  if (*d) MX_TRY(hipFree(*d));
+  *d = nullptr;
+  size_t c = std::max<size_t>(need, 2 * *cap);
+  MX_TRY(hipMalloc(reinterpret_cast<void**>(d), c * sizeof(BlockMap)));
+  *cap = c;
+  return 0;
+}
+
+int ensure_staging(mx_session* S, size_t need) {
+  if (need <= S->h_bm_cap) return 0;
+  MX_TRY(hipStreamSynchronize(S->stream));
+  if (S->h_bm) MX_TRY(hipHostFree(S->h_bm));
+  S->h_bm = nullptr;
+  size_t c = std::max<size_t>(need, 2 * S->h_bm_cap);
+  MX_TRY(hipHostMalloc(reinterpret_cast<void**>(&S->h_bm), c * sizeof(BlockMap), hipHostMallocDefault));
+  S->h_bm_cap = c;
+  return 0;
+}
+
+// Batch view of a session (shares its device tables) for the enqueue helpers.
+Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const BlockMap* bm_vec, int nb_vec) {
+  Batch B;
+  B.mode = S->o.mode;
+  B.ty = S->ty;
+  B.d_ops32 = S->d_ops32;
+  B.d_ops64 = S->d_ops64;
+  B.d_sys = S->d_sys;
+  B.d_bmap_apply = const_cast<BlockMap*>(bm_apply);
+  B.d_bmap_vec = const_cast<BlockMap*>(bm_vec);
+  B.nb_apply = nb_apply;
+  B.nb_vec = nb_vec;
+  return B;
+}
+
+void release_view(Batch& B) {   // the view owns nothing
+  B.d_mem = nullptr;
+  B.stream = nullptr;
+  B.ev[0] = B.ev[1] = nullptr;
+  B.h_state[0] = B.h_state[1] = nullptr;
+}
+
+int session_rebuild(mx_session* S) {
+  std::vector<BlockMap> a, v;
+  for (int i = 0; i < S->capacity; ++i) {
+    if (!S->active[i]) continue;
+    for (int b = 0; b < S->h_sys[i].nblk_apply; ++b) a.push_back({i, b});
+    for (int b = 0; b < S->h_sys[i].nblk_vec; ++b) v.push_back({i, b});
+  }
+  a = xcd_order(a);
+  v = xcd_order(v);
+  S->nb_apply = (int)a.size();
+  S->nb_vec = (int)v.size();
+  const size_t need = a.size() + v.size();
+  if (ensure_device(&S->d_bm, &S->bm_cap, std::max<size_t>(need, 1), S->stream)) return -1;
+  if (ensure_staging(S, std::max<size_t>(need, 1))) return -1;
+  MX_TRY(hipStreamSynchronize(S->stream));   // staging buffer reuse
+  std::copy(a.begin(), a.end(), S->h_bm);
+  std::copy(v.begin(), v.end(), S->h_bm + a.size());
+  if (need) MX_TRY(hipMemcpyAsync(S->d_bm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice, S->stream));
+  if (S->gexec) { MX_TRY(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
+  if (S->graph) { MX_TRY(hipGraphDestroy(S->graph)); S->graph = nullptr; }
+  S->dirty = false;
+  return 0;
+}
+
+}  // namespace
+
+mx_session* mx_session_create(int32_t capacity, const mx_options* opt) {
+  if (capacity < 1) { fail("capacity must be >= 1"); return nullptr; }
+  mx_session* S = new (std::nothrow) mx_session();
+  if (!S) { fail("out of host memory"); return nullptr; }
+  S->o = defaults(opt);
+  S->capacity = capacity;
+  S->ty = S->o.tile_rows ? S->o.tile_rows : 4;
+  S->h_sys.assign(capacity, SysDesc{});
+  S->h_ops.assign(capacity, mx_operator{});
+  S->active.assign(capacity, 0);
+  const size_t s32 = align_up(sizeof(OpView<float>) * capacity);
+  const size_t s64 = align_up(sizeof(OpView<double>) * capacity);
+  const size_t ss = align_up(sizeof(SysDesc) * capacity);
+  hipError_t e = hipMalloc(&S->d_tab, s32 + s64 + ss);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_state), sizeof(SysState) * capacity, hipHostMallocDefault);
+  if (e != hipSuccess) { fail(std::string("mx_session_create: ") + hipGetErrorString(e)); delete S; return nullptr; }
+  S->d_ops32 = reinterpret_cast<OpView<float>*>(S->d_tab);
+  S->d_ops64 = reinterpret_cast<OpView<double>*>(S->d_tab + s32);
+  S->d_sys = reinterpret_cast<SysDesc*>(S->d_tab + s32 + s64);
+  return S;
+}
+
+int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, void* stream) {
+  if (!S || !op || !y) return fail("null argument");
+  std::string why;
+  if (!valid_op(*op, S->o.mode, &why)) return fail(why);
+  if (!y->b || !y->x || !y->work) return fail("b, x and work are required");
+  int slot = -1;
+  for (int i = 0; i < S->capacity; ++i)
+    if (!S->active[i]) { slot = i; break; }
+  if (slot < 0) return fail("session is full");
+  hipStream_t s = S->stream;
+  // caller's inputs (b, x) are produced on the caller's stream
+  MX_TRY(hipEventRecord(S->ev, reinterpret_cast<hipStream_t>(stream)));
+  MX_TRY(hipStreamWaitEvent(s, S->ev, 0));
+
+  const WorkLayout w = work_layout(*op, S->o.mode);
+  char* wk = reinterpret_cast<char*>(y->work);
+  SysDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.op = slot;   // one operator per slot
+  d.slab = S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty);
+  d.nblk_apply = apply_blocks(*op, S->ty, d.slab);
+  d.n3 = 3 * ncell_of(*op);
+  d.nblk_vec = vec_blocks(d.n3);
+  d.vec[VS_X] = y->x;
+  d.vec[VS_B] = const_cast<void*>(y->b);
+  d.vec[VS_R] = wk + w.off_r;
+  d.vec[VS_P] = wk + w.off_p;
+  d.vec[VS_Q] = wk + w.off_q;
+  d.vec[VS_XLO] = wk + w.off_xlo;
+  d.st = reinterpret_cast<SysState*>(wk + w.off_state);
+  d.partial = reinterpret_cast<double*>(wk + w.off_partial);
+  S->h_sys[slot] = d;
+  S->h_ops[slot] = *op;
+
+  // tables and initial state, staged through pinned memory owned by the session
+  MX_TRY(hipStreamSynchronize(s));
+  SysState* st = &S->h_state[slot];
+  std::memset(st, 0, sizeof(*st));
+  st->flags = SF_PAUSE | SF_RESTART;
+  st->rtol = y->rtol;
+  st->atol = y->atol;
+  st->maxiter = std::max(1, y->maxiter);
+  st->delta2 = S->o.delta * S->o.delta;
+  st->stall_iters = std::max(0, S->o.stall_iters);
+  st->stall_limit = std::max(0, S->o.stall_limit);
+  OpView<double> v64 = view_of<double>(*op);
+  OpView<float> v32;
+  if (S->o.mode == MX_MIXED) v32 = view_of<float>(*op);
+  else std::memset(&v32, 0, sizeof(v32));
+  MX_TRY(hipMemcpy(S->d_ops32 + slot, &v32, sizeof(v32), hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(S->d_ops64 + slot, &v64, sizeof(v64), hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(S->d_sys + slot, &d, sizeof(d), hipMemcpyHostToDevice));
+  MX_TRY(hipMemcpy(d.st, st, sizeof(*st), hipMemcpyHostToDevice));
+
+  // admission kernels on this slot only: x_lo = 0, ||b||, r = b - A x0
+  std::vector<BlockMap> a, v;
+  for (int b = 0; b < d.nblk_apply; ++b) a.push_back({slot, b});
+  for (int b = 0; b < d.nblk_vec; ++b) v.push_back({slot, b});
+  a = xcd_order(a);
+  v = xcd_order(v);
+  const size_t need = a.size() + v.size();
+  if (ensure_device(&S->d_adm, &S->adm_cap, need, s)) return -1;
+  if (ensure_staging(S, need)) return -1;
+  std::copy(a.begin(), a.end(), S->h_bm);
+  std::copy(v.begin(), v.end(), S->h_bm + a.size());
+  MX_TRY(hipMemcpy(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
+  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size());
+  int rc = 0;
+  if (launch_zero_slot(S->o.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) rc = fail("zero x_lo");
+  if (!rc && launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s) != hipSuccess) rc = fail("bnorm");
+  if (!rc) rc = enqueue_replacement(B, s);
+  release_view(B);
+  if (rc) return rc;
+  S->active[slot] = 1;
+  S->dirty = true;
+  return slot;
+}
+
+int mx_session_step(mx_session* S, int32_t iterations, void* stream) {
+  if (!S) return fail("null session");
+  (void)stream;
+  if (S->dirty && session_rebuild(S)) return -1;
+  if (S->nb_apply == 0 || iterations <= 0) return 0;
+  Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec);
+  const int check = std::min<int>(S->o.check_every, iterations);
+  int rc = 0;
+  if (S->o.use_graph) {
+    if (!S->gexec || S->graph_iters != iterations) {
+      if (S->gexec) { (void)hipGraphExecDestroy(S->gexec); S->gexec = nullptr; }
+      if (S->graph) { (void)hipGraphDestroy(S->graph); S->graph = nullptr; }
+      hipError_t e = hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) rc = fail(std::string("capture: ") + hipGetErrorString(e));
+      if (!rc) {
+        rc = enqueue_chunk(B, S->stream, iterations, check);
+        e = hipStreamEndCapture(S->stream, &S->graph);
+        if (!rc && e != hipSuccess) rc = fail(std::string("end capture: ") + hipGetErrorString(e));
+      }
+      if (!rc) {
+        e = hipGraphInstantiate(&S->gexec, S->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(std::string("instantiate: ") + hipGetErrorString(e));
+        S->graph_iters = iterations;
+      }
+    }
+    if (!rc) {
+      hipError_t e = hipGraphLaunch(S->gexec, S->stream);
+      if (e != hipSuccess) rc = fail(std::string("graph launch: ") + hipGetErrorString(e));
+    }
+  } else {
+    rc = enqueue_chunk(B, S->stream, iterations, check);
+  }
+  release_view(B);
+  return rc;
+}
+
+int mx_session_poll(mx_session* S, mx_stats* stats, int32_t* done, void* stream) {
+  if (!S) return fail("null session");
+  for (int i = 0; i < S->capacity; ++i)
+    if (S->active[i]) MX_TRY(hipMemcpyAsync(&S->h_state[i], S->h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost, S->stream));
+  MX_TRY(hipStreamSynchronize(S->stream));
+  for (int i = 0; i < S->capacity; ++i) {
+    if (stats) {
+      if (S->active[i]) fill_stats(S->h_state[i], &stats[i]);
+      else std::memset(&stats[i], 0, sizeof(mx_stats));
+    }
+    if (done) done[i] = S->active[i] ? ((S->h_state[i].flags & SF_DONE) ? 1 : 0) : -1;
+  }
+  // results become visible to work the caller queues afterwards on its stream
+  MX_TRY(hipEventRecord(S->ev, S->stream));
+  MX_TRY(hipStreamWaitEvent(reinterpret_cast<hipStream_t>(stream), S->ev, 0));
+  return 0;
+}
+
+int mx_session_retire(mx_session* S, int32_t slot) {
+  if (!S || slot < 0 || slot >= S->capacity || !S->active[slot]) return fail("bad slot");
+  MX_TRY(hipStreamSynchronize(S->stream));
+  S->active[slot] = 0;
+  S->dirty = true;
+  return 0;
+}
+
+void mx_session_destroy(mx_session* S) {
+  if (!S) return;
+  (void)hipStreamSynchronize(S->stream);
+  delete S;
+}
+
+}  // extern "C"

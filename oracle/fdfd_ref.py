@@ -1,0 +1,326 @@
+"""
+FP64 CPU restatement of meanas' 3D FDFD solve path -- TEST INFRASTRUCTURE ONLY
+(used by `tests/`, `__graft_entry__.smoke()` and `bench.py`'s cpu_baseline leg
+as the checker; never imported by the `meanas_amd` product package).
+
+Restated from the reference's text (paths relative to the reference root):
+  meanas/fdmath/vectorization.py:51-125   vec / unvec (C order, component-major)
+  meanas/fdmath/operators.py:15-52        shift_circ
+  meanas/fdmath/operators.py:99-150       deriv_forward / deriv_back
+  meanas/fdmath/operators.py:153-171      cross
+  meanas/fdmath/operators.py:222-251      curl_forward / curl_back
+  meanas/fdfd/operators.py:40-96          e_full
+  meanas/fdfd/operators.py:99-123         e_full_preconditioners
+  meanas/fdfd/operators.py:253-282        e2h
+  meanas/fdfd/operators.py:310-336        poynting_e_cross
+  meanas/fdfd/scpml.py:18-38,98-167       prepare_s_function / stretch_with_scpml
+  meanas/fdfd/solvers.py:21-141           _scipy_qmr / generic
+Third-party arithmetic: scipy.sparse (CSR products) and scipy.sparse.linalg.qmr
+(reference pins scipy~=1.14, pyproject.toml:44; this container has 1.15.3).
+
+How each function is pinned (tests/test_oracle.py):
+  shift_circ        -- equals the permutation numpy.roll performs (textbook routine),
+                       negative distance is the inverse permutation.
+  deriv_*           -- closed form on a periodic complex exponential
+                       ((e^{ik dx}-1)/dx factor, backward (1-e^{-ik dx})/dx).
+  cross / curl_*    -- discrete div(curl)=0 identity (backward divergence of the
+                       forward curl vanishes); plane-wave eigenvalues of curl_b curl_f
+                       (transverse: sum_a (2/dx_a sin(k_a dx_a/2))^2, longitudinal: 0).
+  e_full            -- plane-wave closed form with eps, 1/mu scaling; PEC/PMC rows and
+                       columns vanish exactly; mu divides (checked by uniform mu=2 giving
+                       half the curl-curl term).
+  e_full_precond.   -- Pl A0 Pr is complex-symmetric (A^T = A) on random stretched
+                       grids with anisotropic eps/mu (fails for a wrong index in p).
+  stretch_with_scpml-- closed form of the polynomial stretch on a uniform grid.
+  generic           -- dense numpy.linalg.solve brute force on 5x5x5 grids (forward
+                       and adjoint, the latter against the conjugate-transposed dense
+                       system); the restated reference tests test_residual and
+                       test_poynting_planes (meanas/test/test_fdfd.py:14-55); the 1D
+                       discrete Green's function through a PML-terminated line.
+  e2h / poynting    -- pinned through test_poynting_planes (energy balance).
+Deviation, by necessity: the adjoint branch uses `.conj().T` where the reference
+writes `.H`, an attribute scipy sparse arrays do not have (solvers.py:121).
+"""
+from __future__ import annotations
+
+import logging
+from typing import Any, Callable, Sequence
+
+import numpy
+from scipy import sparse
+import scipy.sparse.linalg
+
+logger = logging.getLogger(__name__)
+
+
+# ---------------------------------------------------------------------------
+# vectorization  (meanas/fdmath/vectorization.py:51-125)
+# ---------------------------------------------------------------------------
+def vec(f):
+    """Flatten a (3, X, Y, Z) field in C order (vectorization.py:51-68)."""
+    if f is None:
+        return None
+    return numpy.ravel(f, order='C')
+
+
+def unvec(v, shape: Sequence[int], nvdim: int = 3):
+    """Inverse of `vec` (vectorization.py:103-125)."""
+    if v is None:
+        return None
+    return v.reshape((nvdim, *shape), order='C')
+
+
+def _grid_shape(dx_lists) -> tuple[int, ...]:
+    return tuple(int(numpy.size(d)) for d in dx_lists)
+
+
+# ---------------------------------------------------------------------------
+# discrete calculus  (meanas/fdmath/operators.py)
+# ---------------------------------------------------------------------------
+def shift_circ(axis: int, shape: Sequence[int], shift_distance: int = 1) -> sparse.csr_array:
+    """Periodic shift operator (fdmath/operators.py:15-52).
+
+    Row r of the operator selects the grid point whose coordinate along `axis` is
+    (coord + |shift_distance|) mod n; a negative distance returns the transpose
+    (the inverse shift).
+    """
+    shape = tuple(int(s) for s in shape)
+    if len(shape) not in (2, 3):
+        raise Exception(f'Invalid shape: {shape}')
+    if axis not in range(len(shape)):
+        raise Exception(f'Invalid direction: {axis}, shape is {shape}')
+    n = int(numpy.prod(shape))
+    flat = numpy.arange(n).reshape(shape)
+    source = numpy.roll(flat, -abs(shift_distance), axis=axis).ravel()
+    op = sparse.csr_array((numpy.ones(n), (numpy.arange(n), source)), shape=(n, n))
+    if shift_distance < 0:
+        op = op.T.tocsr()
+    return op
+
+
+def _expand(dx: Sequence[numpy.ndarray], axis: int) -> numpy.ndarray:
+    """Broadcast the 1D widths of `axis` over the grid, flattened in C order."""
+    grids = numpy.meshgrid(*dx, indexing='ij')
+    return grids[axis].ravel(order='C')
+
+
+def deriv_forward(dx_e: Sequence[numpy.ndarray]) -> list[sparse.csr_array]:
+    """Forward differences (f[i+1]-f[i])/dx_e[i], periodic (fdmath/operators.py:99-123)."""
+    shape = _grid_shape(dx_e)
+    n = int(numpy.prod(shape))
+    eye = sparse.eye_array(n)
+    return [sparse.diags_array(1 / _expand(dx_e, a)) @ (shift_circ(a, shape, 1) - eye)
+            for a in range(len(shape))]
+
+
+def deriv_back(dx_h: Sequence[numpy.ndarray]) -> list[sparse.csr_array]:
+    """Backward differences (f[i]-f[i-1])/dx_h[i], periodic (fdmath/operators.py:126-150)."""
+    shape = _grid_shape(dx_h)
+    n = int(numpy.prod(shape))
+    eye = sparse.eye_array(n)
+    return [sparse.diags_array(-1 / _expand(dx_h, a)) @ (shift_circ(a, shape, -1) - eye)
+            for a in range(len(shape))]
+
+
+def cross(B: Sequence[sparse.sparray]) -> sparse.csr_array:
+    """Block operator of (B x .) (fdmath/operators.py:153-171)."""
+    bx, by, bz = B
+    return sparse.block_array([[None, -bz, by],
+                               [bz, None, -bx],
+                               [-by, bx, None]], format='csr')
+
+
+def curl_forward(dx_e: Sequence[numpy.ndarray]) -> sparse.csr_array:
+    """Curl acting on E (fdmath/operators.py:222-235)."""
+    return cross(deriv_forward(dx_e))
+
+
+def curl_back(dx_h: Sequence[numpy.ndarray]) -> sparse.csr_array:
+    """Curl acting on H (fdmath/operators.py:238-251)."""
+    return cross(deriv_back(dx_h))
+
+
+# ---------------------------------------------------------------------------
+# FDFD operators  (meanas/fdfd/operators.py)
+# ---------------------------------------------------------------------------
+def _mask_keep(mask, n: int):
+    """diag(1 where mask == 0 else 0), identity when mask is None."""
+    if mask is None:
+        return sparse.eye_array(n)
+    return sparse.diags_array(numpy.where(numpy.asarray(mask) != 0, 0.0, 1.0))
+
+
+def e_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None) -> sparse.csr_array:
+    """Wave operator pe (ch pm mu^-1 ce - omega^2 eps) pe (fdfd/operators.py:40-96)."""
+    epsilon = numpy.asarray(epsilon)
+    n = epsilon.size
+    ch = curl_back(dxes[1])
+    ce = curl_forward(dxes[0])
+    pe = _mask_keep(pec, n)
+    pm = _mask_keep(pmc, n)
+    m_div = sparse.eye_array(n) if mu is None else sparse.diags_array(1 / numpy.asarray(mu))
+    op = pe @ (ch @ pm @ m_div @ ce - omega ** 2 * sparse.diags_array(epsilon)) @ pe
+    return op.tocsr()
+
+
+def e_full_preconditioner_vector(dxes) -> numpy.ndarray:
+    """p = sqrt(p^2) with p^2 the per-component products of widths (fdfd/operators.py:116-120)."""
+    (ex, ey, ez), (hx, hy, hz) = dxes
+    p_squared = [ex[:, None, None] * hy[None, :, None] * hz[None, None, :],
+                 hx[:, None, None] * ey[None, :, None] * hz[None, None, :],
+                 hx[:, None, None] * hy[None, :, None] * ez[None, None, :]]
+    return numpy.sqrt(vec(numpy.array(p_squared, dtype=complex)))
+
+
+def e_full_preconditioners(dxes) -> tuple[sparse.sparray, sparse.sparray]:
+    """(Pl, Pr) = (diag(p), diag(1/p)) (fdfd/operators.py:99-123)."""
+    p = e_full_preconditioner_vector(dxes)
+    return sparse.diags_array(p), sparse.diags_array(1 / p)
+
+
+def e2h(omega: complex, dxes, mu=None, pmc=None) -> sparse.csr_array:
+    """E -> H conversion, curl_f / (-i omega) then 1/mu then the PMC mask (fdfd/operators.py:253-282)."""
+    op = curl_forward(dxes[0]) / (-1j * omega)
+    if mu is not None:
+        op = sparse.diags_array(1 / numpy.asarray(mu)) @ op
+    if pmc is not None:
+        op = sparse.diags_array(numpy.where(numpy.asarray(pmc) != 0, 0.0, 1.0)) @ op
+    return op.tocsr()
+
+
+def poynting_e_cross(e, dxes) -> sparse.csr_array:
+    """(E x) part of the Poynting vector with face-area weights (fdfd/operators.py:310-336)."""
+    shape = _grid_shape(dxes[0])
+    fx, fy, fz = (shift_circ(a, shape, 1) for a in range(3))
+    dxa = [d.ravel(order='C') for d in numpy.meshgrid(*dxes[0], indexing='ij')]
+    dxb = [d.ravel(order='C') for d in numpy.meshgrid(*dxes[1], indexing='ij')]
+    ex, ey, ez = (c * w for c, w in zip(numpy.split(numpy.asarray(e), 3), dxa))
+    dg = sparse.diags_array
+    # each block is the diagonal of a shifted (width-weighted) E component
+    blocks = [[None, dg(fx @ -ez), dg(fx @ ey)],
+              [dg(fy @ ez), None, dg(fy @ -ex)],
+              [dg(fz @ -ey), dg(fz @ ex), None]]
+    return (sparse.block_array(blocks) @ dg(numpy.concatenate(dxb))).tocsr()
+
+
+# ---------------------------------------------------------------------------
+# SC-PML  (meanas/fdfd/scpml.py)
+# ---------------------------------------------------------------------------
+def prepare_s_function(ln_R: float = -16, m: float = 4) -> Callable[[numpy.ndarray], numpy.ndarray]:
+    """s(d) = (m+1) ln_R / 2 * d^m (scpml.py:18-38)."""
+    s_max = (m + 1) * ln_R / 2
+
+    def s_factor(distance):
+        return s_max * distance ** m
+    return s_factor
+
+
+def stretch_with_scpml(dxes, axis: int, polarity: int, omega: float,
+                       epsilon_effective: float = 1.0, thickness: int = 10,
+                       s_function=None):
+    """Stretch one face of the grid with an SC-PML (scpml.py:98-167); mutates and returns dxes."""
+    if s_function is None:
+        s_function = prepare_s_function()
+    dx_e = numpy.array(dxes[0][axis], dtype=complex)
+    dx_h = numpy.array(dxes[1][axis], dtype=complex)
+
+    edges = numpy.concatenate(([0], numpy.cumsum(dx_e)))
+    centers = 0.5 * (edges[:-1] + edges[1:])
+    lefts = edges[:-1]
+    correction = numpy.sqrt(epsilon_effective) * numpy.real(omega)
+
+    if polarity > 0:
+        bound = edges[thickness]
+        depth = bound - edges[0]
+        region = slice(thickness)
+
+        def frac(u):
+            return (bound - u) / (bound - edges[0])
+    else:
+        bound = edges[-thickness - 1]
+        depth = edges[-1] - bound
+        region = slice(None) if thickness == 0 else slice(-thickness, None)
+
+        def frac(u):
+            return (u - bound) / (edges[-1] - bound)
+
+    dx_e[region] *= 1 + 1j * s_function(frac(centers[region])) / depth / correction
+    dx_h[region] *= 1 + 1j * s_function(frac(lefts[region])) / depth / correction
+    dxes[0][axis] = dx_e
+    dxes[1][axis] = dx_h
+    return dxes
+
+
+# ---------------------------------------------------------------------------
+# solver  (meanas/fdfd/solvers.py)
+# ---------------------------------------------------------------------------
+def scipy_qmr(A, b, **kwargs: Any) -> numpy.ndarray:
+    """scipy QMR with the reference's every-100-iterations residual log (solvers.py:21-63).
+
+    Returns the iterate whether or not it converged; the info code is dropped as in
+    the reference.  The last info code is kept in `scipy_qmr.last_info` for tests.
+    """
+    count = 0
+
+    def log_residual(xk):
+        nonlocal count
+        count += 1
+        if count % 100 == 0:
+            res = numpy.linalg.norm(A @ xk - b) / numpy.linalg.norm(b)
+            logger.info('Solver residual at iteration %d : %g', count, res)
+
+    user_cb = kwargs.get('callback')
+    if user_cb is not None:
+        def both(xk):
+            log_residual(xk)
+            user_cb(xk)
+        kwargs['callback'] = both
+    else:
+        kwargs['callback'] = log_residual
+    x, info = scipy.sparse.linalg.qmr(A, b, **kwargs)
+    scipy_qmr.last_info = info
+    scipy_qmr.last_iterations = count
+    return x
+
+
+scipy_qmr.last_info = None
+scipy_qmr.last_iterations = 0
+
+
+def generic(omega: complex, dxes, J, epsilon, mu=None, *, pec=None, pmc=None,
+            adjoint: bool = False, matrix_solver: Callable[..., Any] = scipy_qmr,
+            matrix_solver_opts: dict | None = None, E_guess=None) -> numpy.ndarray:
+    """FDFD solve through a sparse matrix solver (solvers.py:66-141)."""
+    if matrix_solver_opts is None:
+        matrix_solver_opts = {}
+    b0 = -1j * omega * numpy.asarray(J)
+    A0 = e_full(omega, dxes, epsilon=epsilon, mu=mu, pec=pec, pmc=pmc)
+    Pl, Pr = e_full_preconditioners(dxes)
+    if adjoint:
+        A = (Pl @ A0 @ Pr).conj().T
+        b = Pr.conj().T @ b0
+    else:
+        A = Pl @ A0 @ Pr
+        b = Pl @ b0
+    if E_guess is not None:
+        x0 = (Pr.conj().T @ E_guess) if adjoint else (Pl @ E_guess)
+        matrix_solver_opts['x0'] = x0
+    x = matrix_solver(A.tocsr(), b, **matrix_solver_opts)
+    return (Pl.conj().T @ x) if adjoint else (Pr @ x)
+
+
+def dense_solve(omega: complex, dxes, J, epsilon, mu=None, pec=None, pmc=None,
+                adjoint: bool = False) -> numpy.ndarray:
+    """Brute-force dense solve of the same system, for tiny grids only.
+
+    PEC rows/columns of e_full are zero; like a Krylov solver started from zero,
+    the minimum-norm solution leaves those entries at zero (lstsq).
+    """
+    A0 = e_full(omega, dxes, epsilon=epsilon, mu=mu, pec=pec, pmc=pmc).toarray()
+    b0 = -1j * omega * numpy.asarray(J)
+    if adjoint:
+        Pl, Pr = (m.toarray() for m in e_full_preconditioners(dxes))
+        A = (Pl @ A0 @ Pr).conj().T
+        z = numpy.linalg.lstsq(A, Pr.conj().T @ b0, rcond=None)[0]
+        return Pl.conj().T @ z
+    return numpy.linalg.lstsq(A0, b0, rcond=None)[0]

@@ -1,0 +1,198 @@
+"""GPU parity of the generic() drop-in against the FP64 oracle.
+
+  * tiny grids: E from the GPU equals the oracle's dense brute-force solve
+    (relative L2 error <= 1e-8 at rtol 1e-11), forward and adjoint, with mu, PEC, PMC;
+  * the reference's own tests restated on the GPU result over the reference fixture
+    grid (meanas/test/test_fdfd.py:14-55): residual < 1e-10, Poynting balance rtol 1e-6;
+  * 48^3 plain workload: relative residual <= 1e-8 (scaled and physical, FP64 oracle
+    CSR) and field error <= 1e-6 against the committed oracle solution;
+  * batching: a system solved alone and inside a batch gives bit-identical E;
+  * failure handling: maxiter exhaustion, zero source, non-finite epsilon.
+"""
+import os
+
+import numpy
+import pytest
+
+from oracle import fdfd_ref as ref
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+
+
+def small_problem(shape, seed, pec_frac=0.0, pmc_frac=0.0, with_mu=False, pml=2):
+    g = numpy.random.default_rng(seed)
+    omega = 2 * numpy.pi / 9.5
+    dxes = [[numpy.ones(n) for n in shape] for _ in range(2)]
+    for a, n in enumerate(shape):
+        if n > 2 * pml + 1:
+            for p in (-1, 1):
+                ref.stretch_with_scpml(dxes, axis=a, polarity=p, omega=omega, thickness=pml)
+    n3 = 3 * int(numpy.prod(shape))
+    eps = g.uniform(1, 4, n3) + 1j * g.uniform(0, 0.05, n3)
+    mu = g.uniform(1, 1.5, n3) if with_mu else None
+    pec = (g.uniform(size=n3) < pec_frac).astype(float) if pec_frac else None
+    pmc = (g.uniform(size=n3) < pmc_frac).astype(float) if pmc_frac else None
+    J = g.standard_normal(n3) + 1j * g.standard_normal(n3)
+    if pec is not None:
+        J[pec != 0] = 0
+    return omega, dxes, J, eps, mu, pec, pmc
+
+
+def rel(a, b):
+    return numpy.linalg.norm(a - b) / numpy.linalg.norm(b)
+
+
+@pytest.mark.parametrize('shape', [(5, 5, 5), (5, 5, 1), (5, 1, 5), (1, 5, 5), (6, 7, 9)])
+@pytest.mark.parametrize('variant', ['plain', 'mu_pmc', 'pec'])
+@pytest.mark.parametrize('adjoint', [False, True])
+@pytest.mark.parametrize('method', ['cocg_mixed', 'cocg_fp64'])
+def test_generic_matches_dense(shape, variant, adjoint, method):
+    from meanas_amd.fdfd import solvers
+    kw = dict(plain={}, mu_pmc=dict(with_mu=True, pmc_frac=0.1), pec=dict(pec_frac=0.1))[variant]
+    omega, dxes, J, eps, mu, pec, pmc = small_problem(shape, 21, **kw)
+    E = solvers.generic(omega, dxes, J, eps, mu, pec=pec, pmc=pmc, adjoint=adjoint, matrix_solver=method,
+                        matrix_solver_opts=dict(rtol=1e-11, atol=0))
+    assert solvers.last_info['status'] == 'converged', solvers.last_info
+    E_dense = ref.dense_solve(omega, dxes, J, eps, mu, pec=pec, pmc=pmc, adjoint=adjoint)
+    assert rel(E, E_dense) < 1e-8
+
+
+def _fixture_cases():
+    cases = []
+    for shape in [(5, 5, 1), (5, 1, 5), (5, 5, 5)]:
+        is3d = 1 not in shape
+        for eps_kind in ['center', '000', 'random']:
+            for bg in [1.0, 1.5]:
+                for fg in [1.0, 2.5]:
+                    if is3d and (eps_kind == '000' or bg != 1 or fg not in (1.0, 2.0)):
+                        continue
+                    for dx in [1.0, 1.5]:
+                        for dxk in ['uniform', 'centerbig']:
+                            cases.append((shape, eps_kind, bg, fg, dx, dxk))
+    return cases
+
+
+@pytest.mark.parametrize('case', _fixture_cases()[::3])
+def test_reference_tests_on_gpu_solution(case):
+    """meanas/test/test_fdfd.py:14-55 with the GPU generic() in place of the reference's."""
+    from meanas_amd.fdfd import solvers
+    shape, eps_kind, bg, fg, dx, dxk = case
+    prng = numpy.random.RandomState(12345)
+    full = (3, *shape)
+    eps = numpy.full(full, bg, dtype=float)
+    if eps_kind == 'center':
+        eps[:, shape[0] // 2, shape[1] // 2, shape[2] // 2] = fg
+    elif eps_kind == '000':
+        eps[:, 0, 0, 0] = fg
+    else:
+        eps[:] = prng.uniform(low=min(bg, fg), high=max(bg, fg), size=full)
+    dxes = [[numpy.full(s, dx) for s in shape] for _ in range(2)]
+    if dxk == 'centerbig':
+        for eh in (0, 1):
+            for ax in (0, 1, 2):
+                dxes[eh][ax][dxes[eh][ax].size // 2] *= 1.1
+    j = numpy.zeros(full, dtype=complex)
+    center = numpy.zeros(full, dtype=bool)
+    center[:, shape[0] // 2, shape[1] // 2, shape[2] // 2] = True
+    j[numpy.roll(center, [1, 1, 1], axis=(1, 2, 3))] = 1 + 1j
+    j[numpy.roll(center, [-1, -1, -1], axis=(1, 2, 3))] = 1 - 1j
+    omega = 1 / 1500
+    ev = solvers.generic(J=ref.vec(j), omega=omega, dxes=dxes, epsilon=ref.vec(eps),
+                         matrix_solver_opts=dict(atol=1e-15, rtol=1e-11))
+    e = ref.unvec(ev, shape)
+    A = ref.e_full(omega, dxes, ref.vec(eps))
+    assert numpy.linalg.norm(A @ ev + 1j * omega * ref.vec(j)) < 1e-10
+    mask = (j != 0).any(axis=0)
+    pts = numpy.where(mask)
+    mask[pts[0][0], pts[1][0], pts[2][0]] = False
+    mx, my, mz = (numpy.roll(mask, -1, axis=a) for a in range(3))
+    hv = ref.e2h(omega, dxes) @ ev
+    s = ref.unvec((ref.poynting_e_cross(ev, dxes) @ hv.conj()).real / 2, shape)
+    planes = [s[0, mask].sum(), -s[0, mx].sum(), s[1, mask].sum(), -s[1, my].sum(),
+              s[2, mask].sum(), -s[2, mz].sum()]
+    dv = dxes[0][0][:, None, None] * dxes[0][1][None, :, None] * dxes[0][2][None, None, :]
+    src = -((e * j.conj()).real * dv)[:, mask] / 2
+    numpy.testing.assert_allclose(sum(planes), src.sum(), rtol=1e-6)
+
+
+@pytest.mark.parametrize('method', ['cocg_mixed', 'cocg_fp64'])
+def test_plain_48_against_golden(method):
+    from meanas_amd import workloads
+    from meanas_amd.fdfd import solvers
+    w = workloads.plain_48()
+    gold = numpy.load(os.path.join(GOLDEN, 'plain_48.npz'))
+    E = solvers.generic(w['omega'], w['dxes'], w['J'], w['epsilon'], matrix_solver=method,
+                        matrix_solver_opts=dict(rtol=1e-8))
+    info = dict(solvers.last_info)
+    assert info['status'] == 'converged' and info['rel_residual'] <= 1e-8, info
+    # physical residual with the oracle's CSR operator (examples/fdfd0.py:90-92)
+    A0 = ref.e_full(w['omega'], w['dxes'], w['epsilon'])
+    b0 = -1j * w['omega'] * w['J']
+    phys = numpy.linalg.norm(A0 @ E - b0) / numpy.linalg.norm(b0)
+    assert phys < 1e-7, phys
+    err = rel(E, gold['E'].astype(numpy.complex128))
+    print(f'{method}: iterations {info["iterations"]}, replacements {info["replacements"]}, '
+          f'scaled residual {info["rel_residual"]:.3e}, physical {phys:.3e}, field error {err:.3e}')
+    assert err <= 1e-6
+
+
+def test_batch_equals_alone_bitwise():
+    """A system's result does not depend on what else is in the batch."""
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    probs = [small_problem(s, 30 + i) for i, s in enumerate([(9, 10, 70), (12, 6, 20), (7, 7, 7)])]
+    ops, rhs = [], []
+    for omega, dxes, J, eps, mu, pec, pmc in probs:
+        op = DeviceOperator(omega, dxes, eps)
+        ops.append(op)
+        rhs.append(op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega))
+    def run(idx):
+        xs = [torch.zeros_like(rhs[i]) for i in idx]
+        systems = [dict(op=k, b=rhs[i], x=xs[k], rtol=1e-10) for k, i in enumerate(idx)]
+        st = solve_batch([ops[i] for i in idx], systems)
+        return [x.cpu().numpy() for x in xs], st
+    alone = [run([i])[0][0] for i in range(3)]
+    mixed, st = run([2, 0, 1])
+    for k, i in enumerate([2, 0, 1]):
+        assert st[k]['status'] == 'converged'
+        assert numpy.array_equal(mixed[k], alone[i])
+
+
+def test_maxiter_returns_last_iterate_with_status():
+    from meanas_amd.fdfd import solvers
+    omega, dxes, J, eps, *_ = small_problem((6, 7, 9), 40)
+    E = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(rtol=1e-12, maxiter=3))
+    assert solvers.last_info['status'] == 'maxiter'
+    assert solvers.last_info['iterations'] == 3
+    assert numpy.isfinite(E).all() and numpy.linalg.norm(E) > 0
+
+
+def test_zero_source_gives_zero_field():
+    from meanas_amd.fdfd import solvers
+    omega, dxes, J, eps, *_ = small_problem((6, 7, 9), 41)
+    E = solvers.generic(omega, dxes, numpy.zeros_like(J), eps)
+    assert not E.any()
+
+
+def test_nonfinite_epsilon_is_reported():
+    from meanas_amd.fdfd import solvers
+    omega, dxes, J, eps, *_ = small_problem((6, 7, 9), 42)
+    eps = eps.copy()
+    eps[17] = numpy.nan
+    solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(rtol=1e-10))
+    assert solvers.last_info['status'] == 'nonfinite'
+
+
+def test_e_guess_semantics():
+    from meanas_amd.fdfd import solvers
+    omega, dxes, J, eps, *_ = small_problem((6, 7, 9), 43)
+    E = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(rtol=1e-11))
+    opts = dict(rtol=1e-11)
+    E2 = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=opts, E_guess=E)
+    # the scaled guess is written back as the reference does (fdfd/solvers.py:127-132)
+    numpy.testing.assert_allclose(opts['x0'], ref.e_full_preconditioner_vector(dxes) * E, rtol=1e-12)
+    assert solvers.last_info['iterations'] <= 2
+    assert rel(E2, E) < 1e-9

@@ -1,0 +1,100 @@
+"""CPU checks of the product boundary: the C-ABI library loads and exports exactly the
+entry points include/meanas_fdfd.h declares, the ctypes prototypes cover them, the
+struct layouts agree with the header, and the host-side PML / coefficient preparation
+matches the oracle.  No compute call is made (no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+
+from oracle import fdfd_ref as ref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'meanas_fdfd.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(mx_[a-z_0-9]+)\s*\(', text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from meanas_amd import _lib
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.PROTOTYPES), set(names) ^ set(_lib.PROTOTYPES)
+    assert L.mx_abi_version() == 1
+
+
+def test_struct_sizes_match_header_layout():
+    from meanas_amd import _lib
+    # mx_operator: 4 int32 + 2 double + 7 pointers ; mx_system: 2 int32 + 2 double + 3 pointers
+    assert ctypes.sizeof(_lib.MxOperator) == 16 + 16 + 7 * 8
+    assert ctypes.sizeof(_lib.MxSystem) == 8 + 16 + 3 * 8
+    assert ctypes.sizeof(_lib.MxOptions) == 6 * 4 + 8 + 2 * 4
+    assert ctypes.sizeof(_lib.MxStats) == 16 + 16
+
+
+def test_work_bytes_is_a_pure_host_query():
+    from meanas_amd import _lib
+    op = _lib.MxOperator(32, 16, 70, 0, 1.0, 0.0, 1, 1, 1, 1, None, None, None)
+    n3 = 3 * 32 * 16 * 70
+    mixed = _lib.lib().mx_work_bytes(ctypes.byref(op), _lib.MX_MIXED)
+    fp64 = _lib.lib().mx_work_bytes(ctypes.byref(op), _lib.MX_FP64)
+    assert mixed >= 4 * 8 * n3 and fp64 >= 4 * 16 * n3
+    assert _lib.lib().mx_work_bytes(ctypes.byref(op), 7) == 0
+
+
+def test_product_scpml_matches_oracle():
+    from meanas_amd.fdfd import scpml
+    shape = (30, 1, 41)
+    omega = 2 * numpy.pi / 38.75
+    mine = [[numpy.linspace(1, 1.2, n) for n in shape] for _ in range(2)]
+    theirs = [[d.copy() for d in row] for row in mine]
+    for a in (0, 2):
+        for p in (-1, 1):
+            scpml.stretch_with_scpml(mine, axis=a, polarity=p, omega=omega, epsilon_effective=2.0, thickness=7)
+            ref.stretch_with_scpml(theirs, axis=a, polarity=p, omega=omega, epsilon_effective=2.0, thickness=7)
+    for eh in (0, 1):
+        for a in range(3):
+            numpy.testing.assert_allclose(mine[eh][a], theirs[eh][a], rtol=1e-15)
+
+
+def test_coefficient_table_reproduces_scaled_operator():
+    """The per-axis table (fa, fb) builds C^T M C - w^2 eps equal to Pl e_full Pr (oracle)."""
+    from scipy import sparse
+    from meanas_amd.fdfd.device import coefficient_table
+    g = numpy.random.default_rng(0)
+    shape = (4, 5, 3)
+    dxes = [[g.uniform(0.7, 1.4, n) + 1j * g.uniform(-0.3, 0.3, n) for n in shape] for _ in range(2)]
+    tab = coefficient_table(dxes)
+    n3 = 3 * int(numpy.prod(shape))
+    eps = g.uniform(1, 3, n3) + 1j * g.uniform(-0.2, 0.2, n3)
+    mu = g.uniform(0.8, 1.5, n3)
+    w = 0.9 - 0.05j
+    offs = numpy.cumsum([0, *shape])
+    D = []
+    for a in range(3):
+        fa, fb = tab[offs[a]:offs[a + 1], 0], tab[offs[a]:offs[a + 1], 1]
+        S = ref.shift_circ(a, shape, 1)
+        grid = lambda v: numpy.meshgrid(*[v if b == a else numpy.ones(n) for b, n in enumerate(shape)],
+                                        indexing='ij')[a].ravel()
+        D.append(sparse.diags_array(grid(fa)) @ S - sparse.diags_array(grid(fb)))
+    C = ref.cross(D)
+    A = C.T @ sparse.diags_array(1 / mu) @ C - w ** 2 * sparse.diags_array(eps)
+    Pl, Pr = ref.e_full_preconditioners(dxes)
+    want = Pl @ ref.e_full(w, dxes, eps, mu=mu) @ Pr
+    assert abs(A - want).max() <= 1e-14 * abs(want).max()
+
+
+def test_generic_rejects_callable_matrix_solver():
+    from meanas_amd.fdfd import solvers
+    dxes = [[numpy.ones(3)] * 3] * 2
+    with pytest.raises(TypeError):
+        solvers.generic(1.0, dxes, numpy.zeros(81), numpy.ones(81), matrix_solver=lambda A, b: b)
