@@ -45,6 +45,13 @@ def parse():
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-iters', type=int, default=20)
+    ap.add_argument('--apply-sweep', action='store_true',
+                    help='only time the operator apply over tile heights and slab depths')
+    ap.add_argument('--apply-only', action='store_true',
+                    help='only run the operator apply (complex64 and complex128, default tiling): '
+                         'the program the hardware-counter profiles are taken on')
+    ap.add_argument('--mode', choices=['mixed', 'fp64'], default='mixed',
+                    help='solver of the timed step (the other one is the same-GPU comparison)')
     return ap.parse_args()
 
 
@@ -72,6 +79,50 @@ def cpu_baseline(w, iters):
     return out
 
 
+def time_apply(plan, x, y, reps, stream):
+    import torch
+    for _ in range(5):
+        plan.apply(x, y, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        plan.apply(x, y, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / reps
+
+
+def apply_sweep(op, dev):
+    """Apply time over tile heights x slab depths, complex64 and complex128."""
+    import torch
+    stream = torch.cuda.current_stream()
+    for mode, dt, elem in (('c64', torch.complex64, 8), ('c128', torch.complex128, 16)):
+        x = torch.randn(op.n3, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        bpc = op.apply_bytes_per_cell(elem)
+        for ty in (2, 4, 8):
+            for slab in (1, 2, 4, 8, 16, 32, 64):
+                plan = op.plan(mode, tile_rows=ty, slab=slab)
+                us = time_apply(plan, x, y, 50, stream)
+                print(json.dumps(dict(mode=mode, tile_rows=ty, slab=slab, blocks=plan.blocks, us=round(us, 2),
+                                      gbs=round(bpc * op.ncell / (us * 1e-6) / 1e9, 1))), flush=True)
+        torch.cuda.synchronize()
+    # plain copy of the same bytes for reference (HBM roofline as the hardware delivers it)
+    nbytes = op.apply_bytes_per_cell(8) * op.ncell // 2
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    c = torch.empty_like(a)
+    for _ in range(5):
+        c.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(50):
+        c.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / 50
+    print(json.dumps(dict(copy_bytes=2 * nbytes, us=round(us, 2), gbs=round(2 * nbytes / (us * 1e-6) / 1e9, 1))))
+
+
 def main():
     args = parse()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -89,6 +140,27 @@ def main():
     w = workloads.boxes_128(args.size)
     log(f'workload {w["name"]} shape {w["shape"]}')
     op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
+    if args.apply_sweep:
+        apply_sweep(op, dev)
+        return
+    if args.apply_only:
+        stream = torch.cuda.current_stream()
+        for mode, dt, elem in (('c64', torch.complex64, 8), ('c128', torch.complex128, 16)):
+            xa = torch.randn(op.n3, dtype=dt, device=dev)
+            ya = torch.empty_like(xa)
+            us = time_apply(op.plan(mode), xa, ya, args.apply_reps, stream)
+            bpc = op.apply_bytes_per_cell(elem)
+            print(json.dumps(dict(mode=mode, us=round(us, 2), bytes_per_cell=bpc,
+                                  gbs=round(bpc * op.ncell / (us * 1e-6) / 1e9, 1))), flush=True)
+        # a plain copy of known size: reference for scaling the FETCH_SIZE / WRITE_SIZE counters
+        nb = 64 << 20
+        a = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        c = torch.empty_like(a)
+        for _ in range(args.apply_reps):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes_each_way=nb, reps=args.apply_reps)), flush=True)
+        return
     J = torch.from_numpy(w['J']).to(dev)
     b = op.map(_lib.MX_MAP_RHS, J, coef=-1j * w['omega'])
     x = torch.zeros_like(b)
@@ -100,9 +172,10 @@ def main():
         return solve_batch([op], [dict(op=0, b=b, x=xout, rtol=rtol, atol=0.0, maxiter=maxiter, work=work[mode])],
                            mode=mode)[0]
 
-    # ---- mixed-precision solves: the measured step
+    step_mode = _lib.MX_MIXED if args.mode == 'mixed' else _lib.MX_FP64
+    # ---- the measured step: complete solves
     for i in range(args.warmup):
-        st = solve(_lib.MX_MIXED, args.rtol, x)
+        st = solve(step_mode, args.rtol, x)
         log(f'warmup {i}: {st}')
     torch.cuda.synchronize()
     lat = []
@@ -110,7 +183,7 @@ def main():
     t_start = time.perf_counter()
     for i in range(args.steps):
         t0 = time.perf_counter()
-        st = solve(_lib.MX_MIXED, args.rtol, x)
+        st = solve(step_mode, args.rtol, x)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0)
         stats.append(st)
@@ -133,10 +206,10 @@ def main():
         E_ref = op.map(_lib.MX_MAP_E, x64)
         ferr = float(torch.linalg.vector_norm(E_mixed - E_ref) / torch.linalg.vector_norm(E_ref))
         log(f'fp64 reference solve (1e-11): {st_ref}; field error of mixed solve {ferr:.3e}')
-        extra.update(fp64_cocg=dict(solves_per_s=1.0 / t64, ms=t64 * 1e3, iterations=st64['iterations'],
-                                    rel_residual=st64['rel_residual']),
-                     mixed_over_fp64_speedup=(1.0 / numpy.median(lat)) / (1.0 / t64),
-                     field_error_vs_fp64_1e-11=ferr)
+        extra.update({'fp64_cocg': dict(solves_per_s=1.0 / t64, ms=t64 * 1e3, iterations=st64['iterations'],
+                                        rel_residual=st64['rel_residual']),
+                      'mixed_over_fp64_speedup': (1.0 / numpy.median(lat)) / (1.0 / t64),
+                      'field_error_vs_fp64_rtol_1e-11': ferr})
         del x64
 
     # ---- apply kernel roofline: HIP events on the stream the kernel is launched on

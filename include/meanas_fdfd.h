@@ -90,10 +90,13 @@ typedef struct mx_options {
   int32_t check_every; /* iterations between residual-replacement checks (0 = automatic) */
   int32_t use_graph;   /* 1: replay hipGraph-captured iteration chunks, 0: plain launches */
   double delta;        /* replacement trigger: recursive ||r|| < delta * max ||r|| (0 = automatic) */
-  int32_t stall_iters; /* force an FP64 residual replacement after this many iterations without one
-                          (0 = automatic: 1000; < 0 = never) */
-  int32_t stall_limit; /* stop with MX_STALLED after this many consecutive replacements that do not
-                          halve the true residual (0 = automatic: 4; < 0 = never) */
+  int32_t stall_iters; /* stall window: an FP64 residual replacement is forced after this many
+                          iterations without one, and the solve stops with MX_STALLED when its best
+                          true residual has not halved within this many (0 = automatic: 2000;
+                          < 0 = never) */
+  int32_t stall_limit; /* stop with MX_STALLED after this many consecutive replacements at which the
+                          recursive residual had drifted >= 100x below the true one (0 = automatic: 3;
+                          < 0 = never) */
 } mx_options;
 
 typedef struct mx_stats {
@@ -132,11 +135,15 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
  *   MX_MAP_RHS_ADJ  out = conj(c * in) / p          conj of Pr^H (-i w J) (adjoint)
  *   MX_MAP_E        out = in / p                    E = Pr x
  *   MX_MAP_E_ADJ    out = conj(p * in)              E = Pl^H z, z = conj(x) (adjoint)
+ *   MX_MAP_SIGN     out = s * (c * in),  s = +-1    the factored p times s is the reference's
+ *                                                  principal root sqrt(p^2) (operators.py:120);
+ *                                                  maps a scaled vector between the two conventions
  * With c = 1, MX_MAP_RHS / MX_MAP_RHS_ADJ also scale an initial guess E_guess. */
 #define MX_MAP_RHS 0
 #define MX_MAP_RHS_ADJ 1
 #define MX_MAP_E 2
 #define MX_MAP_E_ADJ 3
+#define MX_MAP_SIGN 4
 int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const void* in, void* out, void* stream);
 
 /* Solve a batch of systems (one or many right-hand sides, one or many operators of

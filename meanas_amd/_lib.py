@@ -31,6 +31,7 @@ MX_MAP_RHS = 0
 MX_MAP_RHS_ADJ = 1
 MX_MAP_E = 2
 MX_MAP_E_ADJ = 3
+MX_MAP_SIGN = 4
 
 c_void_p = ctypes.c_void_p
 

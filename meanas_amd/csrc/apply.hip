@@ -52,7 +52,6 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   __shared__ uint8_t sM[2][LY][LZ];
   __shared__ cplx<T> sH[3][TY + 1][TZ + 1];
   __shared__ double s_red[4 * 16];
-  __shared__ int s_flag;
 
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
@@ -100,23 +99,25 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   const int col_lds = col_r * LZ + (col_side ? TZ + 1 : 0);
   const int nc = (int)ncell;
 
-  cplx<T> pre_o[3], pre_r[3], pre_c[3];
-  uint8_t m_o = 0, m_r = 0, m_c = 0;
-
-  auto fetch = [&](int ip) {
+  // One plane's worth of this thread's loads (own column, y-halo row, z-halo column).
+  struct Regs {
+    cplx<T> o[3], r[3], c[3];
+    uint8_t mo, mr, mc;
+  };
+  auto fetch = [&](Regs& R, int ip) {
     const int pb = ip * (int)plane;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pre_o[c] = load_c<T>(xin, c * nc + pb + own_off);
-    if (has_mask) m_o = op.mask[pb + own_off];
+    for (int c = 0; c < 3; ++c) R.o[c] = load_c<T>(xin, c * nc + pb + own_off);
+    R.mo = has_mask ? op.mask[pb + own_off] : 0;
     if (has_row) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pre_r[c] = load_c<T>(xin, c * nc + pb + row_off);
-      if (has_mask) m_r = op.mask[pb + row_off];
+      for (int c = 0; c < 3; ++c) R.r[c] = load_c<T>(xin, c * nc + pb + row_off);
+      R.mr = has_mask ? op.mask[pb + row_off] : 0;
     }
     if (has_col) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) pre_c[c] = load_c<T>(xin, c * nc + pb + col_off);
-      if (has_mask) m_c = op.mask[pb + col_off];
+      for (int c = 0; c < 3; ++c) R.c[c] = load_c<T>(xin, c * nc + pb + col_off);
+      R.mc = has_mask ? op.mask[pb + col_off] : 0;
     }
   };
   auto put = [&](int buf, int lds, const cplx<T> (&v)[3], uint8_t m) {
@@ -125,11 +126,13 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     for (int c = 0; c < 3; ++c) base[c * LY * LZ + lds] = (m >> c) & 1 ? zero<T>() : v[c];
     (&sM[buf][0][0])[lds] = m;
   };
-  auto commit = [&](int buf) {
-    put(buf, own_lds, pre_o, m_o);
-    if (has_row) put(buf, row_lds, pre_r, m_r);
-    if (has_col) put(buf, col_lds, pre_c, m_c);
+  auto commit = [&](const Regs& R, int buf) {
+    put(buf, own_lds, R.o, R.mo);
+    if (has_row) put(buf, row_lds, R.r, R.mr);
+    if (has_col) put(buf, col_lds, R.c, R.mc);
   };
+  // Register prefetch: the loads of plane i+2 are in flight while plane i is computed.
+  Regs RA;
 
   // per-thread y/z coefficients (wrapped indices)
   const AxisCoef<T> cyj = cy[jw], czk = cz[kw];
@@ -151,10 +154,10 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   cplx<T> hpy, hpz;
   {
     const int im1 = wrapi(ib - 1, nx);
-    fetch(im1);
-    commit(0);
-    fetch(ib);
-    commit(1);
+    fetch(RA, im1);
+    commit(RA, 0);
+    fetch(RA, ib);
+    commit(RA, 1);
     __syncthreads();
     const AxisCoef<T> cxm = cx[im1];
     const int r = ty + 1, q = tz + 1;
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     const int cell = im1 * (int)plane + own_off;
     hpy = hscale(hy, 1, m, cell);
     hpz = hscale(hz, 2, m, cell);
-    fetch(wrapi(ib + 1, nx));                     // plane ib+1, committed by the first step
+    fetch(RA, wrapi(ib + 1, nx));                 // plane ib+1, committed by the first step
     __syncthreads();
   }
 
@@ -179,10 +182,10 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
 
-  for (int i = ib; i < ie; ++i) {
+  auto step = [&](int i, Regs& R) {
     const int nb = cb ^ 1;
-    commit(nb);                                   // plane i+1
-    if (i + 1 < ie) fetch(wrapi(i + 2, nx));      // prefetch plane i+2
+    commit(R, nb);                                // plane i+1
+    if (i + 1 < ie) fetch(R, wrapi(i + 2, nx));   // prefetch plane i+2
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
@@ -271,71 +274,12 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
       }
     }
     cb = nb;
-  }
+  };
+  for (int i = ib; i < ie; ++i) step(i, RA);
 
-  if constexpr (EPI == EPI_PLAIN) return;
-  else {
-    const int nblk = S.nblk_apply;
+  if constexpr (EPI != EPI_PLAIN) {
     block_sum<3>(acc, s_red);
-    const bool last = publish_partials<3>(acc, S.partial, bm.local, nblk, &st->counter[EPI == EPI_COCG ? 0 : 2], &s_flag);
-    if (!last) return;
-    double tot[3];
-    final_sum<3>(tot, S.partial, nblk, s_red);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-      st->counter[EPI == EPI_COCG ? 0 : 2] = 0;
-      int f = st->flags;
-      if constexpr (EPI == EPI_COCG) {
-        if (f & SF_MR) {
-          // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
-          // new residual r - alpha A g generic (r^T r no longer ~ 0).  SF_RESTART stays
-          // set: COCG restarts from the new residual.
-          const double qq = tot[2];
-          const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
-          if (!(qq > 0.0) || !finite(alpha.re)) {
-            f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0: no progress possible
-          } else {
-            st->alpha = alpha;
-          }
-        } else {
-          const cplx<double> mu{tot[0], tot[1]};
-          st->mu = mu;
-          const cplx<double> alpha = cdiv(st->rho, mu);
-          if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
-            // p^T A p vanished (or overflowed): replace the residual, then rescue with a
-            // minimal-residual step and restart from p = r
-            f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
-            if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
-          } else {
-            st->alpha = alpha;
-            f &= ~SF_RESTART;
-          }
-        }
-      } else {   // EPI_RESID: residual replacement / initial residual
-        const double rn2 = tot[0];
-        const cplx<double> rho{tot[1], tot[2]};
-        st->true_rnorm2 = rn2;
-        st->rnorm2 = rn2;
-        st->rmax2 = rn2;
-        st->replacements += 1;
-        st->since = 0;
-        // stagnation: replacements that no longer halve the true residual norm
-        if (st->replacements > 1 && rn2 > 0.25 * st->best_true2) st->stalls += 1;
-        else st->stalls = 0;
-        if (st->replacements == 1 || rn2 < st->best_true2) st->best_true2 = rn2;
-        if (!finite(rn2)) f |= SF_NAN | SF_DONE;
-        if (st->stall_limit > 0 && st->stalls >= st->stall_limit) f |= SF_STALL | SF_DONE;
-        if (rn2 <= st->stop2 || (f & (SF_DONE | SF_LIMIT))) {
-          f |= SF_DONE;
-        } else {
-          const cplx<double> ro = st->rho_old;
-          st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
-          st->rho = rho;
-          if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2) f |= SF_RESTART | SF_MR;
-          f &= ~(SF_PAUSE | SF_BREAKDOWN);
-        }
-      }
-      st->flags = f;
-    }
+    store_partials<3>(acc, S.partial, bm.local, S.nblk_apply);
   }
 }
 

@@ -80,13 +80,14 @@ enum : int {
   SF_LIMIT     = 128,  // iteration limit reached: the next replacement finishes the system
   SF_MR        = 256,  // next step is a rescue step along a hashed direction (breakdown)
   SF_STALL     = 512,  // stopped: replacements no longer reduce the true residual
+  SF_SAVE      = 1024, // the last replacement found the best true residual: save x
 };
 
 // |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown
 constexpr double kRhoBreakdown = 1e-10;
 
-// Scalars of one linear system, device resident.  Written by the last block of each
-// reduction kernel (deterministic fixed-order sums), read by the following kernels.
+// Scalars of one linear system, device resident.  Written by the finishing kernel that
+// follows each reducing kernel (deterministic fixed-order sums), read by later kernels.
 struct SysState {
   cplx<double> rho;        // r^T r (unconjugated) of the current residual
   cplx<double> rho_old;    // previous rho (for beta after a residual replacement)
@@ -99,20 +100,25 @@ struct SysState {
   double stop2;            // recursive-residual target (absolute, squared)
   double delta2;           // reliable-update trigger: pause when rnorm2 < delta2 * rmax2
   double rtol, atol;       // convergence: ||b - A x|| <= max(rtol ||b||, atol)
-  double best_true2;       // smallest true ||r||^2 seen at a replacement
+  double best_true2;       // smallest true ||r||^2 seen at a replacement (x saved in VS_XBEST)
+  double rnorm_rec2;       // recursive ||r||^2 just before the last replacement
+  double mark2;            // true ||r||^2 at the last 2x improvement of the best residual
   int flags;
   int iters;               // inner iterations performed
   int replacements;        // residual replacements (reliable updates / refinements)
   int maxiter;
   int since;               // iterations since the last replacement
-  int stall_iters;         // force a replacement after this many iterations without one
-  int stalls;              // consecutive replacements without a 2x true-residual gain
-  int stall_limit;         // stop (SF_STALL) after this many
-  unsigned int counter[4]; // last-block arrival counters (apply, update, residual, norm)
+  int stall_iters;         // stall window: a replacement is forced after this many
+                           // iterations without one, and the solve stops (SF_STALL) when
+                           // the best true residual has not halved within this many
+  int stalls;              // consecutive replacements whose recursive residual had drifted
+                           // >= 100x below the true one (precision exhausted)
+  int stall_limit;         // stop (SF_STALL) after this many such replacements
+  int mark_iter;           // iteration of the last 2x improvement
 };
 
 // One linear system of a batch.  Vector slots name the vectors a kernel reads/writes.
-enum : int { VS_X = 0, VS_R, VS_P, VS_Q, VS_XLO, VS_B, VS_T, VS_COUNT };
+enum : int { VS_X = 0, VS_R, VS_P, VS_Q, VS_XLO, VS_B, VS_XBEST, VS_COUNT };
 
 struct SysDesc {
   int op;                     // operator index
@@ -159,35 +165,40 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* scratch) {
   __syncthreads();
 }
 
-// Publish this block's K partial sums; returns true in every thread of the block that
-// arrived last for this system.  That block then owns the fixed-order final sum.
+// Store this block's K partial sums (plain stores by thread 0).  No fence: the partials
+// are consumed by a separate finishing kernel, after the kernel boundary has made them
+// visible on every XCD.  (A per-block device-scope release fence would write back the
+// XCD's whole L2 on this part and costs more than the kernel itself.)
 template <int K>
-__device__ __forceinline__ bool publish_partials(const double (&v)[K], double* partial, int local,
-                                                 int nblk, unsigned int* counter, int* s_flag) {
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  if (tid == 0) {
+__device__ __forceinline__ void store_partials(const double (&v)[K], double* partial, int local, int nblk) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
 #pragma unroll
     for (int q = 0; q < K; ++q) partial[q * nblk + local] = v[q];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    *s_flag = (prev == (unsigned int)(nblk - 1));
   }
-  __syncthreads();
-  bool last = *s_flag != 0;
-  if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return last;
 }
 
-// In the last block: fixed-order sum of the K partial arrays; result in thread 0.
+// Fixed-order sum of the K partial arrays of one system; result in thread 0.  Each
+// thread first issues all of its loads (at most kFinLoads per array), then sums.
+constexpr int kFinThreads = 1024;
+constexpr int kFinLoads = 4;   // nblk <= kFinThreads * kFinLoads partials per array
 template <int K>
 __device__ __forceinline__ void final_sum(double (&out)[K], const double* partial, int nblk, double* scratch) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int nthreads = blockDim.x * blockDim.y;
+  double v[K][kFinLoads];
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int l = 0; l < kFinLoads; ++l) {
+      const int b = tid + l * nthreads;
+      v[q][l] = b < nblk ? partial[q * nblk + b] : 0.0;
+    }
 #pragma unroll
   for (int q = 0; q < K; ++q) {
-    double s = 0.0;
-    for (int b = tid; b < nblk; b += nthreads) s += partial[q * nblk + b];
-    out[q] = s;
+    double sacc = 0.0;
+#pragma unroll
+    for (int l = 0; l < kFinLoads; ++l) sacc += v[q][l];
+    out[q] = sacc;
   }
   block_sum<K>(out, scratch);
 }

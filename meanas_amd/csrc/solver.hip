@@ -30,7 +30,10 @@ hipError_t launch_update_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_accum_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_accum_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_bnorm(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_save_best(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_zero_slot(int prec64, const SysDesc*, const BlockMap*, int, int, hipStream_t);
+hipError_t launch_fin(int kind, const SysDesc*, const int* syslist, int nsys, hipStream_t);
+enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3 };
 hipError_t launch_diag(const AxisCoef<double>*, int, int, int, const void*, void*, double, double, int, hipStream_t);
 }  // namespace mx
 
@@ -55,6 +58,7 @@ constexpr int kVecBlock = 256;
 constexpr int kVecElemsPerThread = 8;
 constexpr int kVecMaxBlocks = 2048;
 constexpr size_t kAlign = 256;
+constexpr int kMaxPartials = kFinThreads * kFinLoads;   // per-system partials one finishing block sums
 
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -89,14 +93,17 @@ OpView<T> view_of(const mx_operator& op) {
   return v;
 }
 
-int auto_tile_rows(const mx_operator& op) { return op.ny >= 8 ? 4 : 2; }
+// The automatic tile height is fixed: a system's block partition (and with it the order
+// of every reduction) must depend on that system alone, never on the rest of a batch.
+constexpr int kAutoTileRows = 4;
 
 int auto_slab(const mx_operator& op, int ty) {
   // enough blocks to fill 256 CUs several times over, but deep slabs to amortise the
-  // two extra planes a slab reads
+  // two extra planes a slab reads; never more partials than one finishing block sums
   const long long tiles = (long long)((op.ny + ty - 1) / ty) * ((op.nz + 63) / 64);
   int slab = 16;
   while (slab > 4 && tiles * ((op.nx + slab - 1) / slab) < 2048) slab /= 2;
+  while (tiles * ((op.nx + slab - 1) / slab) > kMaxPartials) slab *= 2;
   return std::min(slab, op.nx);
 }
 
@@ -116,9 +123,9 @@ int vec_blocks(long long n3) {
 
 size_t elem_bytes(int mode) { return mode == MX_MIXED ? 8 : 16; }
 
-// workspace layout per system: r, p, q, x_lo (inner precision), state, partials
+// workspace layout per system: r, p, q, x_lo (inner precision), x_best (FP64), state, partials
 struct WorkLayout {
-  size_t off_r, off_p, off_q, off_xlo, off_state, off_partial, total;
+  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, total;
 };
 
 WorkLayout work_layout(const mx_operator& op, int mode) {
@@ -129,7 +136,8 @@ WorkLayout work_layout(const mx_operator& op, int mode) {
   w.off_p = w.off_r + vb;
   w.off_q = w.off_p + vb;
   w.off_xlo = w.off_q + vb;
-  w.off_state = w.off_xlo + vb;
+  w.off_xbest = w.off_xlo + vb;
+  w.off_state = w.off_xbest + align_up((size_t)n3 * 16);
   w.off_partial = w.off_state + align_up(sizeof(SysState));
   const size_t nbmax = (size_t)std::max(apply_blocks_upper(op), kVecMaxBlocks);
   w.total = w.off_partial + align_up(4 * nbmax * sizeof(double));
@@ -166,6 +174,8 @@ struct Batch {
   SysDesc* d_sys = nullptr;
   BlockMap* d_bmap_apply = nullptr;
   BlockMap* d_bmap_vec = nullptr;
+  int* d_syslist = nullptr;   // systems the finishing kernels visit (one block each)
+  int nsl = 0;
   std::vector<SysDesc> h_sys;
   SysState* h_state[2] = {nullptr, nullptr};   // pinned, double-buffered polls
   hipStream_t stream = nullptr;                  // library-owned capture stream
@@ -189,13 +199,7 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
   B.mode = o.mode;
   B.nops = nops;
   B.nsys = nsys;
-  int ty = o.tile_rows;
-  if (ty == 0) {
-    int mny = 1 << 30;
-    for (int i = 0; i < nops; ++i) mny = std::min(mny, ops[i].ny);
-    ty = auto_tile_rows(ops[0]);
-    if (mny < 8) ty = 2;
-  }
+  const int ty = o.tile_rows ? o.tile_rows : kAutoTileRows;
   if (ty != 2 && ty != 4 && ty != 8) return fail("tile_rows must be 2, 4 or 8");
   B.ty = ty;
   for (int i = 0; i < nops; ++i)
@@ -217,12 +221,16 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.nblk_apply = apply_blocks(op, ty, d.slab);
     d.n3 = 3 * ncell_of(op);
     d.nblk_vec = vec_blocks(d.n3);
+    if (d.nblk_apply > kMaxPartials)
+      return fail("system " + std::to_string(s) + ": " + std::to_string(d.nblk_apply) +
+                  " apply blocks exceed the reduction limit; use a deeper slab");
     d.vec[VS_X] = y.x;
     d.vec[VS_B] = const_cast<void*>(y.b);
     d.vec[VS_R] = wk + w.off_r;
     d.vec[VS_P] = wk + w.off_p;
     d.vec[VS_Q] = wk + w.off_q;
     d.vec[VS_XLO] = wk + w.off_xlo;
+    d.vec[VS_XBEST] = wk + w.off_xbest;
     d.st = reinterpret_cast<SysState*>(wk + w.off_state);
     d.partial = reinterpret_cast<double*>(wk + w.off_partial);
     B.h_sys[s] = d;
@@ -239,13 +247,19 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
   const size_t s_sys = align_up(sizeof(SysDesc) * nsys);
   const size_t s_ba = align_up(sizeof(BlockMap) * bm_apply.size());
   const size_t s_bv = align_up(sizeof(BlockMap) * bm_vec.size());
-  MX_TRY(hipMalloc(&B.d_mem, s_ops32 + s_ops64 + s_sys + s_ba + s_bv));
+  const size_t s_sl = align_up(sizeof(int) * nsys);
+  MX_TRY(hipMalloc(&B.d_mem, s_ops32 + s_ops64 + s_sys + s_ba + s_bv + s_sl));
   char* p = B.d_mem;
   B.d_ops32 = reinterpret_cast<OpView<float>*>(p); p += s_ops32;
   B.d_ops64 = reinterpret_cast<OpView<double>*>(p); p += s_ops64;
   B.d_sys = reinterpret_cast<SysDesc*>(p); p += s_sys;
   B.d_bmap_apply = reinterpret_cast<BlockMap*>(p); p += s_ba;
-  B.d_bmap_vec = reinterpret_cast<BlockMap*>(p);
+  B.d_bmap_vec = reinterpret_cast<BlockMap*>(p); p += s_bv;
+  B.d_syslist = reinterpret_cast<int*>(p);
+  B.nsl = nsys;
+  std::vector<int> sl(nsys);
+  for (int i = 0; i < nsys; ++i) sl[i] = i;
+  MX_TRY(hipMemcpy(B.d_syslist, sl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
 
   std::vector<OpView<float>> v32(nops);
   std::vector<OpView<double>> v64(nops);
@@ -272,12 +286,15 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
   if (mixed) {
     MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
     MX_TRY(launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
     MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
     MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   }
+  MX_TRY(launch_fin(FIN_UPDATE, B.d_sys, B.d_syslist, B.nsl, s));
   return 0;
 }
 
@@ -290,6 +307,8 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
     MX_TRY(launch_accum_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
     MX_TRY(launch_apply_resid_r64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
   }
+  MX_TRY(launch_fin(FIN_RESID, B.d_sys, B.d_syslist, B.nsl, s));
+  MX_TRY(launch_save_best(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   return 0;
 }
 
@@ -302,12 +321,20 @@ int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
   return 0;
 }
 
+// A finished system that did not converge returns its best iterate (restored by
+// restore_best); its reported residual is that iterate's.
+bool wants_best(const SysState& st) {
+  return (st.flags & SF_DONE) && !(st.true_rnorm2 <= st.stop2) && st.best_true2 < st.true_rnorm2 &&
+         !(st.flags & SF_NAN);
+}
+
 void fill_stats(const SysState& st, mx_stats* out) {
   out->iterations = st.iters;
   out->replacements = st.replacements;
   out->flags = st.flags;
   out->bnorm = std::sqrt(st.bnorm2);
-  out->rel_residual = st.bnorm2 > 0 ? std::sqrt(st.true_rnorm2 / st.bnorm2) : std::sqrt(st.true_rnorm2);
+  const double r2 = wants_best(st) ? st.best_true2 : st.true_rnorm2;
+  out->rel_residual = st.bnorm2 > 0 ? std::sqrt(r2 / st.bnorm2) : std::sqrt(r2);
   if (st.flags & SF_NAN) out->status = MX_NONFINITE;
   else if (st.true_rnorm2 <= st.stop2) out->status = MX_CONVERGED;
   else if (st.flags & SF_STALL) out->status = MX_STALLED;
@@ -322,8 +349,8 @@ mx_options defaults(const mx_options* opt) {
   if (o.chunk <= 0) o.chunk = 16;
   if (o.check_every <= 0) o.check_every = (o.mode == MX_MIXED) ? 1 : o.chunk;
   if (o.delta <= 0) o.delta = (o.mode == MX_MIXED) ? 0.1 : 0.0;
-  if (o.stall_iters == 0) o.stall_iters = 1000;
-  if (o.stall_limit == 0) o.stall_limit = 4;
+  if (o.stall_iters == 0) o.stall_iters = 2000;
+  if (o.stall_limit == 0) o.stall_limit = 3;
   return o;
 }
 
@@ -352,7 +379,7 @@ mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, 
   if (!op) { fail("null operator"); return nullptr; }
   if (mode != MX_MIXED && mode != MX_FP64) { fail("mode must be MX_MIXED or MX_FP64"); return nullptr; }
   if (!valid_op(*op, mode, &why)) { fail(why); return nullptr; }
-  const int ty = tile_rows ? tile_rows : (op->ny >= 8 ? 4 : 2);
+  const int ty = tile_rows ? tile_rows : kAutoTileRows;
   if (ty != 2 && ty != 4 && ty != 8) { fail("tile_rows must be 2, 4 or 8"); return nullptr; }
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
@@ -418,7 +445,7 @@ int mx_apply(const mx_operator* op, int32_t mode, const void* x, void* y, void* 
 
 int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const void* in, void* out, void* stream) {
   if (!op || !in || !out || !op->coef64) return fail("null argument");
-  if (map < 0 || map > 3) return fail("unknown map");
+  if (map < 0 || map > 4) return fail("unknown map");
   MX_TRY(launch_diag(reinterpret_cast<const AxisCoef<double>*>(op->coef64), op->nx, op->ny, op->nz, in, out, c_re, c_im,
                      map, reinterpret_cast<hipStream_t>(stream)));
   return 0;
@@ -429,7 +456,7 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   std::string why;
   if (!valid_op(*op, MX_FP64, &why)) return fail(why);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const int ty = op->ny >= 8 ? 4 : 2;
+  const int ty = kAutoTileRows;
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
   d.slab = auto_slab(*op, ty);
@@ -447,7 +474,8 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   const size_t s_part = align_up(sizeof(double) * 4 * (size_t)d.nblk_apply);
   const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
   char* mem = nullptr;
-  MX_TRY(hipMalloc(&mem, s_op + s_sys + s_st + s_part + s_bm));
+  MX_TRY(hipMalloc(&mem, s_op + s_sys + s_st + s_part + s_bm + align_up(sizeof(int))));
+  int* d_one = reinterpret_cast<int*>(mem + s_op + s_sys + s_st + s_part + s_bm);
   d.st = reinterpret_cast<SysState*>(mem + s_op + s_sys);
   d.partial = reinterpret_cast<double*>(mem + s_op + s_sys + s_st);
   SysState st;
@@ -460,8 +488,11 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   if (e == hipSuccess) e = hipMemcpy(mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(d.st, &st, sizeof(st), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(mem + s_op + s_sys + s_st + s_part, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
+  const int zero_idx = 0;
+  if (e == hipSuccess) e = hipMemcpy(d_one, &zero_idx, sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_apply_resid_r64(ty, reinterpret_cast<const OpView<double>*>(mem), reinterpret_cast<const SysDesc*>(mem + s_op),
                                                  reinterpret_cast<const BlockMap*>(mem + s_op + s_sys + s_st + s_part), d.nblk_apply, s);
+  if (e == hipSuccess) e = launch_fin(FIN_RESID, reinterpret_cast<const SysDesc*>(mem + s_op), d_one, 1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) rc = fail(std::string("mx_residual: ") + hipGetErrorString(e));
@@ -501,6 +532,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   for (int i = 0; i < nsys; ++i) MX_TRY(hipMemcpyAsync(B.h_sys[i].st, &init[i], sizeof(SysState), hipMemcpyHostToDevice, s));
   if (launch_zero_slot(B.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) return fail("zero x_lo");
   MX_TRY(launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  MX_TRY(launch_fin(FIN_BNORM, B.d_sys, B.d_syslist, B.nsl, s));
   if (enqueue_replacement(B, s)) return -1;
 
   const int chunk = o.chunk;
@@ -547,6 +579,11 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   // final states (the queued chunk, if any, was a no-op on finished systems)
   std::vector<SysState> fin(nsys);
   for (int i = 0; i < nsys; ++i) MX_TRY(hipMemcpy(&fin[i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nsys; ++i)
+    if (wants_best(fin[i]))
+      MX_TRY(hipMemcpyAsync(B.h_sys[i].vec[VS_X], B.h_sys[i].vec[VS_XBEST], (size_t)B.h_sys[i].n3 * 16,
+                            hipMemcpyDeviceToDevice, s));
+  MX_TRY(hipStreamSynchronize(s));
   if (stats)
     for (int i = 0; i < nsys; ++i) fill_stats(fin[i], &stats[i]);
   MX_TRY(hipEventRecord(B.ev[0], s));
@@ -571,10 +608,12 @@ struct mx_session {
   size_t bm_cap = 0;
   BlockMap* d_adm = nullptr;             // maps of the slot being admitted
   size_t adm_cap = 0;
+  int* d_syslist = nullptr;              // capacity + 1 ints: active slots, then the admitted slot
+  int nsl = 0;
   std::vector<SysDesc> h_sys;
   std::vector<mx_operator> h_ops;
   std::vector<char> active;
-  std::vector<int> ty_of;                // tile rows used by each slot (all equal here)
+  std::vector<char> restored;            // best iterate already restored for a finished slot
   int ty = 4;
   int nb_apply = 0, nb_vec = 0;
   bool dirty = true;
@@ -625,8 +664,11 @@ int ensure_staging(mx_session* S, size_t need) {
 }
 
 // Batch view of a session (shares its device tables) for the enqueue helpers.
-Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const BlockMap* bm_vec, int nb_vec) {
+Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const BlockMap* bm_vec, int nb_vec,
+                    const int* syslist, int nsl) {
   Batch B;
+  B.d_syslist = const_cast<int*>(syslist);
+  B.nsl = nsl;
   B.mode = S->o.mode;
   B.ty = S->ty;
   B.d_ops32 = S->d_ops32;
@@ -641,6 +683,7 @@ Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const
 
 void release_view(Batch& B) {   // the view owns nothing
   B.d_mem = nullptr;
+  B.d_syslist = nullptr;
   B.stream = nullptr;
   B.ev[0] = B.ev[1] = nullptr;
   B.h_state[0] = B.h_state[1] = nullptr;
@@ -648,11 +691,14 @@ void release_view(Batch& B) {   // the view owns nothing
 
 int session_rebuild(mx_session* S) {
   std::vector<BlockMap> a, v;
+  std::vector<int> sl;
   for (int i = 0; i < S->capacity; ++i) {
     if (!S->active[i]) continue;
+    sl.push_back(i);
     for (int b = 0; b < S->h_sys[i].nblk_apply; ++b) a.push_back({i, b});
     for (int b = 0; b < S->h_sys[i].nblk_vec; ++b) v.push_back({i, b});
   }
+  S->nsl = (int)sl.size();
   a = xcd_order(a);
   v = xcd_order(v);
   S->nb_apply = (int)a.size();
@@ -664,6 +710,7 @@ int session_rebuild(mx_session* S) {
   std::copy(a.begin(), a.end(), S->h_bm);
   std::copy(v.begin(), v.end(), S->h_bm + a.size());
   if (need) MX_TRY(hipMemcpyAsync(S->d_bm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice, S->stream));
+  if (!sl.empty()) MX_TRY(hipMemcpy(S->d_syslist, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
   if (S->gexec) { MX_TRY(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
   if (S->graph) { MX_TRY(hipGraphDestroy(S->graph)); S->graph = nullptr; }
   S->dirty = false;
@@ -678,14 +725,16 @@ mx_session* mx_session_create(int32_t capacity, const mx_options* opt) {
   if (!S) { fail("out of host memory"); return nullptr; }
   S->o = defaults(opt);
   S->capacity = capacity;
-  S->ty = S->o.tile_rows ? S->o.tile_rows : 4;
+  S->ty = S->o.tile_rows ? S->o.tile_rows : kAutoTileRows;
   S->h_sys.assign(capacity, SysDesc{});
   S->h_ops.assign(capacity, mx_operator{});
   S->active.assign(capacity, 0);
+  S->restored.assign(capacity, 0);
   const size_t s32 = align_up(sizeof(OpView<float>) * capacity);
   const size_t s64 = align_up(sizeof(OpView<double>) * capacity);
   const size_t ss = align_up(sizeof(SysDesc) * capacity);
-  hipError_t e = hipMalloc(&S->d_tab, s32 + s64 + ss);
+  const size_t ssl = align_up(sizeof(int) * (capacity + 1));
+  hipError_t e = hipMalloc(&S->d_tab, s32 + s64 + ss + ssl);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&S->ev, hipEventDisableTiming);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&S->h_state), sizeof(SysState) * capacity, hipHostMallocDefault);
@@ -693,6 +742,7 @@ mx_session* mx_session_create(int32_t capacity, const mx_options* opt) {
   S->d_ops32 = reinterpret_cast<OpView<float>*>(S->d_tab);
   S->d_ops64 = reinterpret_cast<OpView<double>*>(S->d_tab + s32);
   S->d_sys = reinterpret_cast<SysDesc*>(S->d_tab + s32 + s64);
+  S->d_syslist = reinterpret_cast<int*>(S->d_tab + s32 + s64 + ss);
   return S;
 }
 
@@ -719,12 +769,14 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   d.nblk_apply = apply_blocks(*op, S->ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   d.nblk_vec = vec_blocks(d.n3);
+  if (d.nblk_apply > kMaxPartials) return fail("too many apply blocks for one system; use a deeper slab");
   d.vec[VS_X] = y->x;
   d.vec[VS_B] = const_cast<void*>(y->b);
   d.vec[VS_R] = wk + w.off_r;
   d.vec[VS_P] = wk + w.off_p;
   d.vec[VS_Q] = wk + w.off_q;
   d.vec[VS_XLO] = wk + w.off_xlo;
+  d.vec[VS_XBEST] = wk + w.off_xbest;
   d.st = reinterpret_cast<SysState*>(wk + w.off_state);
   d.partial = reinterpret_cast<double*>(wk + w.off_partial);
   S->h_sys[slot] = d;
@@ -762,14 +814,18 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   std::copy(a.begin(), a.end(), S->h_bm);
   std::copy(v.begin(), v.end(), S->h_bm + a.size());
   MX_TRY(hipMemcpy(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
-  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size());
+  int* adm_sl = S->d_syslist + S->capacity;
+  MX_TRY(hipMemcpy(adm_sl, &slot, sizeof(int), hipMemcpyHostToDevice));
+  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1);
   int rc = 0;
   if (launch_zero_slot(S->o.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) rc = fail("zero x_lo");
   if (!rc && launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s) != hipSuccess) rc = fail("bnorm");
+  if (!rc && launch_fin(FIN_BNORM, B.d_sys, B.d_syslist, B.nsl, s) != hipSuccess) rc = fail("bnorm finish");
   if (!rc) rc = enqueue_replacement(B, s);
   release_view(B);
   if (rc) return rc;
   S->active[slot] = 1;
+  S->restored[slot] = 0;
   S->dirty = true;
   return slot;
 }
@@ -779,7 +835,7 @@ int mx_session_step(mx_session* S, int32_t iterations, void* stream) {
   (void)stream;
   if (S->dirty && session_rebuild(S)) return -1;
   if (S->nb_apply == 0 || iterations <= 0) return 0;
-  Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec);
+  Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec, S->d_syslist, S->nsl);
   const int check = std::min<int>(S->o.check_every, iterations);
   int rc = 0;
   if (S->o.use_graph) {
@@ -815,6 +871,13 @@ int mx_session_poll(mx_session* S, mx_stats* stats, int32_t* done, void* stream)
   for (int i = 0; i < S->capacity; ++i)
     if (S->active[i]) MX_TRY(hipMemcpyAsync(&S->h_state[i], S->h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost, S->stream));
   MX_TRY(hipStreamSynchronize(S->stream));
+  for (int i = 0; i < S->capacity; ++i)
+    if (S->active[i] && !S->restored[i] && (S->h_state[i].flags & SF_DONE)) {
+      if (wants_best(S->h_state[i]))
+        MX_TRY(hipMemcpyAsync(S->h_sys[i].vec[VS_X], S->h_sys[i].vec[VS_XBEST], (size_t)S->h_sys[i].n3 * 16,
+                              hipMemcpyDeviceToDevice, S->stream));
+      S->restored[i] = 1;
+    }
   for (int i = 0; i < S->capacity; ++i) {
     if (stats) {
       if (S->active[i]) fill_stats(S->h_state[i], &stats[i]);

@@ -20,10 +20,33 @@
 namespace mx {
 
 constexpr int VB = 256;   // threads per vector block
+constexpr int VU = 4;     // 16-byte packs in flight per thread
 
 __device__ __forceinline__ bool skip_sys(int f, int skip, int need) {
   return (f & skip) || (need && !(f & need));
 }
+
+// 16-byte pack of complex values: 2 complex64 or 1 complex128 (one dwordx4 access).
+template <typename T> struct alignas(16) Pack {
+  static constexpr int K = 16 / sizeof(cplx<T>);
+  cplx<T> v[K];
+};
+
+// Contiguous pack range [*b, *e) of this block: the vector is split into nblk equal runs.
+__device__ __forceinline__ void block_range(long long npk, int local, int nblk, long long* b, long long* e) {
+  const long long per = (npk + nblk - 1) / nblk;
+  *b = min(npk, per * local);
+  *e = min(npk, *b + per);
+}
+
+// Pack range of one loop trip of this thread: packs base + u*VB, u < VU, below pe.
+// Every kernel below loads all its in-flight packs first, then computes and stores, so
+// VU loads per vector are outstanding at once.
+struct Trip {
+  long long base, pe;
+  __device__ __forceinline__ bool ok(int u) const { return base + (long long)u * VB < pe; }
+  __device__ __forceinline__ long long at(int u) const { return base + (long long)u * VB; }
+};
 
 // ---- p = r + beta p ------------------------------------------------------------
 template <typename T>
@@ -36,28 +59,50 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
   const cplx<T>* __restrict__ r = reinterpret_cast<const cplx<T>*>(S.vec[VS_R]);
   cplx<T>* __restrict__ p = reinterpret_cast<cplx<T>*>(S.vec[VS_P]);
   const long long n = S.n3;
-  const long long stride = (long long)S.nblk_vec * VB;
   if (f & SF_MR) {   // breakdown rescue: p = hashed direction, zero on PEC entries
     const OpView<T>& op = ops[S.op];
     const long long ncell = n / 3;
+    const long long stride = (long long)S.nblk_vec * VB;
     for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) {
       const int c = (int)(e / ncell);
       const bool pec = op.mask && ((op.mask[e - c * ncell] >> c) & 1);
       p[e] = pec ? zero<T>() : cast<T>(hashed_direction(e));
     }
-  } else if (f & SF_RESTART) {
-    for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) p[e] = r[e];
-  } else {
-    const cplx<T> beta = cast<T>(S.st->beta);
-    for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) p[e] = r[e] + beta * p[e];
+    return;
   }
+  const Pack<T>* __restrict__ r4 = reinterpret_cast<const Pack<T>*>(r);
+  Pack<T>* __restrict__ p4 = reinterpret_cast<Pack<T>*>(p);
+  constexpr int K = Pack<T>::K;
+  long long pb, pe;
+  block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
+  const bool restart = (f & SF_RESTART) != 0;
+  const cplx<T> beta = restart ? zero<T>() : cast<T>(S.st->beta);
+  for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
+    Pack<T> a[VU], c[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+        a[u] = r4[t.at(u)];
+        if (!restart) c[u] = p4[t.at(u)];
+      }
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+        if (!restart) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) a[u].v[k] = a[u].v[k] + beta * c[u].v[k];
+        }
+        p4[t.at(u)] = a[u];
+      }
+  }
+  if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0)
+    p[n - 1] = restart ? r[n - 1] : r[n - 1] + beta * p[n - 1];
 }
 
 // ---- x_lo += alpha p ; r -= alpha q ; rho, ||r||^2 ---------------------------------
 template <typename T>
 __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
   __shared__ double s_red[3 * 16];
-  __shared__ int s_flag;
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   SysState* st = S.st;
@@ -66,55 +111,51 @@ __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ syste
   cplx<T>* __restrict__ r = reinterpret_cast<cplx<T>*>(S.vec[VS_R]);
   const cplx<T>* __restrict__ p = reinterpret_cast<const cplx<T>*>(S.vec[VS_P]);
   const cplx<T>* __restrict__ q = reinterpret_cast<const cplx<T>*>(S.vec[VS_Q]);
+  Pack<T>* __restrict__ x4 = reinterpret_cast<Pack<T>*>(x);
+  Pack<T>* __restrict__ r4 = reinterpret_cast<Pack<T>*>(r);
+  const Pack<T>* __restrict__ p4 = reinterpret_cast<const Pack<T>*>(p);
+  const Pack<T>* __restrict__ q4 = reinterpret_cast<const Pack<T>*>(q);
   const cplx<T> alpha = cast<T>(st->alpha);
   const long long n = S.n3;
-  const long long stride = (long long)S.nblk_vec * VB;
   T rr_re = 0, rr_im = 0, nr = 0;
-  for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) {
-    x[e] = x[e] + alpha * p[e];
-    const cplx<T> rv = r[e] - alpha * q[e];
-    r[e] = rv;
+  auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T> pv, cplx<T> qv) {
+    xv = xv + alpha * pv;
+    rv = rv - alpha * qv;
     rr_re += rv.re * rv.re - rv.im * rv.im;
     rr_im += T(2) * rv.re * rv.im;
     nr += rv.re * rv.re + rv.im * rv.im;
+  };
+  constexpr int K = Pack<T>::K;
+  long long pb, pe;
+  block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
+  for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
+    Pack<T> xv[VU], rv[VU], pv[VU], qv[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+        xv[u] = x4[t.at(u)];
+        rv[u] = r4[t.at(u)];
+        pv[u] = p4[t.at(u)];
+        qv[u] = q4[t.at(u)];
+      }
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
+        x4[t.at(u)] = xv[u];
+        r4[t.at(u)] = rv[u];
+      }
+  }
+  if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
+    cplx<T> xe = x[n - 1], re = r[n - 1];
+    one(xe, re, p[n - 1], q[n - 1]);
+    x[n - 1] = xe;
+    r[n - 1] = re;
   }
   double acc[3] = {(double)rr_re, (double)rr_im, (double)nr};
   block_sum<3>(acc, s_red);
-  const int nblk = S.nblk_vec;
-  if (!publish_partials<3>(acc, S.partial, bm.local, nblk, &st->counter[1], &s_flag)) return;
-  double tot[3];
-  final_sum<3>(tot, S.partial, nblk, s_red);
-  if (threadIdx.x == 0) {
-    st->counter[1] = 0;
-    int f = st->flags;
-    const cplx<double> rho{tot[0], tot[1]};
-    const double rn2 = tot[2];
-    if (f & SF_MR) {   // that was a minimal-residual step: COCG restarts from p = r
-      f &= ~SF_MR;
-      st->beta = cplx<double>{0.0, 0.0};
-    } else if (norm2(st->rho) > 0.0) {
-      st->beta = cdiv(rho, st->rho);
-    } else {   // rho breakdown: restart from the replaced residual
-      f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
-      st->beta = cplx<double>{0.0, 0.0};
-    }
-    // r^T r ~ 0 with r != 0: the next step is a minimal-residual step from p = r
-    if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0) f |= SF_MR | SF_RESTART;
-    st->rho_old = st->rho;
-    st->rho = rho;
-    st->rnorm2 = rn2;
-    if (rn2 > st->rmax2) st->rmax2 = rn2;
-    st->iters += 1;
-    st->since += 1;
-    if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
-      f |= SF_NAN | SF_PAUSE | SF_LIMIT;
-    } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
-               (st->stall_iters > 0 && st->since >= st->stall_iters)) {
-      f |= SF_PAUSE;
-    }
-    if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
-    st->flags = f;
-  }
+  store_partials<3>(acc, S.partial, bm.local, S.nblk_vec);
 }
 
 // ---- x (c128) += x_lo ; x_lo = 0   (paused systems) --------------------------------
@@ -126,37 +167,73 @@ __global__ __launch_bounds__(VB) void k_accum(const SysDesc* __restrict__ system
   cplx<double>* __restrict__ x = reinterpret_cast<cplx<double>*>(S.vec[VS_X]);
   cplx<T>* __restrict__ xl = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   const long long n = S.n3;
-  const long long stride = (long long)S.nblk_vec * VB;
-  for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) {
-    x[e] = x[e] + cast<double>(xl[e]);
-    xl[e] = zero<T>();
+  Pack<T>* __restrict__ xl4 = reinterpret_cast<Pack<T>*>(xl);
+  constexpr int K = Pack<T>::K;
+  long long pb, pe;
+  block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
+  Pack<T> z;
+#pragma unroll
+  for (int k = 0; k < K; ++k) z.v[k] = zero<T>();
+  for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
+    Pack<T> a[VU];
+    cplx<double> xv[VU][K];
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+        a[u] = xl4[t.at(u)];
+#pragma unroll
+        for (int k = 0; k < K; ++k) xv[u][k] = x[t.at(u) * K + k];
+      }
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[t.at(u) * K + k] = xv[u][k] + cast<double>(a[u].v[k]);
+        xl4[t.at(u)] = z;
+      }
+  }
+  if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
+    x[n - 1] = x[n - 1] + cast<double>(xl[n - 1]);
+    xl[n - 1] = zero<T>();
+  }
+}
+
+// ---- x_best = x after a replacement that found a new best true residual -----------------
+__global__ __launch_bounds__(VB) void k_save_best(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
+  const BlockMap bm = bmap[blockIdx.x];
+  const SysDesc& S = systems[bm.sys];
+  if (skip_sys(S.st->flags, SF_DONE | SF_INACTIVE, SF_SAVE)) return;
+  const cplx<double>* __restrict__ x = reinterpret_cast<const cplx<double>*>(S.vec[VS_X]);
+  cplx<double>* __restrict__ xb = reinterpret_cast<cplx<double>*>(S.vec[VS_XBEST]);
+  const long long n = S.n3;
+  long long pb, pe;
+  block_range(n, bm.local, S.nblk_vec, &pb, &pe);
+  for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
+    cplx<double> v[VU];
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) v[u] = x[t.at(u)];
+#pragma unroll
+    for (int u = 0; u < VU; ++u)
+      if (t.ok(u)) xb[t.at(u)] = v[u];
   }
 }
 
 // ---- ||b||^2 and the stopping target, once per solve --------------------------------
 __global__ __launch_bounds__(VB) void k_bnorm(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
   __shared__ double s_red[1 * 16];
-  __shared__ int s_flag;
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   SysState* st = S.st;
   if (skip_sys(st->flags, SF_DONE | SF_INACTIVE, 0)) return;
   const cplx<double>* __restrict__ b = reinterpret_cast<const cplx<double>*>(S.vec[VS_B]);
   const long long n = S.n3;
-  const long long stride = (long long)S.nblk_vec * VB;
   double acc[1] = {0.0};
-  for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) acc[0] += norm2(b[e]);
+  long long pb, pe;
+  block_range(n, bm.local, S.nblk_vec, &pb, &pe);
+  for (long long e = pb + threadIdx.x; e < pe; e += VB) acc[0] += norm2(b[e]);
   block_sum<1>(acc, s_red);
-  const int nblk = S.nblk_vec;
-  if (!publish_partials<1>(acc, S.partial, bm.local, nblk, &st->counter[3], &s_flag)) return;
-  double tot[1];
-  final_sum<1>(tot, S.partial, nblk, s_red);
-  if (threadIdx.x == 0) {
-    st->counter[3] = 0;
-    st->bnorm2 = tot[0];
-    const double a = st->rtol * st->rtol * tot[0], c = st->atol * st->atol;
-    st->stop2 = a > c ? a : c;
-  }
+  store_partials<1>(acc, S.partial, bm.local, S.nblk_vec);
 }
 
 // ---- fill (graph-safe replacement for memset) ---------------------------------------
@@ -167,8 +244,15 @@ __global__ __launch_bounds__(VB) void k_zero_slot(const SysDesc* __restrict__ sy
   if (skip_sys(S.st->flags, SF_DONE | SF_INACTIVE, 0)) return;
   cplx<T>* __restrict__ v = reinterpret_cast<cplx<T>*>(S.vec[slot]);
   const long long n = S.n3;
-  const long long stride = (long long)S.nblk_vec * VB;
-  for (long long e = (long long)bm.local * VB + threadIdx.x; e < n; e += stride) v[e] = zero<T>();
+  Pack<T>* __restrict__ v4 = reinterpret_cast<Pack<T>*>(v);
+  constexpr int K = Pack<T>::K;
+  long long pb, pe;
+  block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
+  Pack<T> z;
+#pragma unroll
+  for (int k = 0; k < K; ++k) z.v[k] = zero<T>();
+  for (long long i = pb + threadIdx.x; i < pe; i += VB) v4[i] = z;
+  if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) v[n - 1] = zero<T>();
 }
 
 // ---- diagonal scaling p of the symmetrised system ------------------------------------
@@ -192,6 +276,9 @@ __device__ __forceinline__ cplx<double> pscale(const AxisCoef<double>* coef, int
 // mode 1: out = (1/p) * conj(coef * in)    (adjoint b = Pr^H(-i w J) conjugated, x0 likewise)
 // mode 2: out = in / p                     (forward E = Pr x)
 // mode 3: out = conj(p * in)               (adjoint E = Pl^H z with z = conj(x))
+// mode 4: out = s * (coef * in), s = +-1 the sign that turns the factored p into the
+//         principal root sqrt(p^2) the reference uses (fdfd/operators.py:120): +1 when
+//         p lies in the principal half-plane (Re p > 0, or Re p = 0 and Im p >= 0).
 __global__ __launch_bounds__(VB) void k_diag(const AxisCoef<double>* __restrict__ coef, int nx, int ny, int nz,
                                             const cplx<double>* __restrict__ in, cplx<double>* __restrict__ out,
                                             double cre, double cim, int mode) {
@@ -205,10 +292,141 @@ __global__ __launch_bounds__(VB) void k_diag(const AxisCoef<double>* __restrict_
       case 0: o = p * (k * v); break;
       case 1: o = cdiv(conj(k * v), p); break;
       case 2: o = cdiv(v, p); break;
-      default: o = conj(p * v); break;
+      case 3: o = conj(p * v); break;
+      default: {
+        const bool principal = p.re > 0.0 || (p.re == 0.0 && p.im >= 0.0);
+        o = principal ? k * v : cplx<double>{-(k * v).re, -(k * v).im};
+        break;
+      }
     }
     out[e] = o;
   }
+}
+
+
+// ---- finishing kernels: one block per system sums that system's partials in a fixed
+// order and advances its scalar state.  They follow the reducing kernel of the same
+// kind and apply the same skip rule, so partials are never read stale.
+enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3 };
+
+template <int KIND>
+__global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
+  __shared__ double s_red[3 * 16];
+  const SysDesc& S = systems[syslist[blockIdx.x]];
+  SysState* st = S.st;
+  int f = st->flags;
+  if (KIND == FIN_APPLY || KIND == FIN_UPDATE) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  } else if (KIND == FIN_RESID) {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
+  } else {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
+  }
+  const int nblk = (KIND == FIN_APPLY || KIND == FIN_RESID) ? S.nblk_apply : S.nblk_vec;
+  double tot[3] = {0.0, 0.0, 0.0};
+  if constexpr (KIND == FIN_BNORM) {
+    double t1[1];
+    final_sum<1>(t1, S.partial, nblk, s_red);
+    tot[0] = t1[0];
+  } else {
+    final_sum<3>(tot, S.partial, nblk, s_red);
+  }
+  if (threadIdx.x != 0) return;
+
+  if constexpr (KIND == FIN_APPLY) {
+    if (f & SF_MR) {
+      // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
+      // new residual r - alpha A g generic (r^T r no longer ~ 0).  SF_RESTART stays set:
+      // COCG restarts from the new residual.
+      const double qq = tot[2];
+      const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
+      if (!(qq > 0.0) || !finite(alpha.re)) f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0
+      else st->alpha = alpha;
+    } else {
+      const cplx<double> mu{tot[0], tot[1]};
+      st->mu = mu;
+      const cplx<double> alpha = cdiv(st->rho, mu);
+      if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
+        // p^T A p vanished (or overflowed): replace the residual, then rescue with a
+        // step along a hashed direction and restart from p = r
+        f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
+        if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
+      } else {
+        st->alpha = alpha;
+        f &= ~SF_RESTART;
+      }
+    }
+  } else if constexpr (KIND == FIN_UPDATE) {
+    const cplx<double> rho{tot[0], tot[1]};
+    const double rn2 = tot[2];
+    f &= ~SF_SAVE;     // any saved best iterate was stored by k_save_best before this
+    if (f & SF_MR) {   // that was a rescue step: COCG restarts from p = r
+      f &= ~SF_MR;
+      st->beta = cplx<double>{0.0, 0.0};
+    } else if (norm2(st->rho) > 0.0) {
+      st->beta = cdiv(rho, st->rho);
+    } else {           // rho breakdown: restart from the replaced residual
+      f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
+      st->beta = cplx<double>{0.0, 0.0};
+    }
+    // r^T r ~ 0 with r != 0: the next step is a rescue step from a hashed direction
+    if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0) f |= SF_MR | SF_RESTART;
+    st->rho_old = st->rho;
+    st->rho = rho;
+    st->rnorm2 = rn2;
+    if (rn2 > st->rmax2) st->rmax2 = rn2;
+    st->iters += 1;
+    st->since += 1;
+    if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
+      f |= SF_NAN | SF_PAUSE | SF_LIMIT;
+    } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
+               (st->stall_iters > 0 && st->since >= st->stall_iters)) {
+      f |= SF_PAUSE;
+    }
+    if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
+  } else if constexpr (KIND == FIN_RESID) {
+    // residual replacement / initial residual
+    const double rn2 = tot[0];
+    const cplx<double> rho{tot[1], tot[2]};
+    st->true_rnorm2 = rn2;
+    st->rnorm_rec2 = st->rnorm2;   // recursive residual the replacement overwrites
+    st->rnorm2 = rn2;
+    st->rmax2 = rn2;
+    st->replacements += 1;
+    st->since = 0;
+    // stagnation: (a) the recursive residual had drifted >= 100x below the true one at
+    // stall_limit consecutive replacements (the inner precision no longer tracks the
+    // operator), or (b) the best true residual has not halved within stall_iters
+    // iterations.  COCG's residual oscillates, so single replacements prove nothing.
+    const bool first = st->replacements == 1;
+    if (!first && st->rnorm_rec2 < 1e-4 * rn2) st->stalls += 1;
+    else st->stalls = 0;
+    if (first || rn2 < 0.25 * st->mark2) {
+      st->mark2 = rn2;
+      st->mark_iter = st->iters;
+    }
+    if (first || rn2 < st->best_true2) {
+      st->best_true2 = rn2;
+      f |= SF_SAVE;
+    }
+    if (st->stall_limit > 0 && st->stalls >= st->stall_limit) f |= SF_STALL | SF_DONE;
+    if (st->stall_iters > 0 && st->iters - st->mark_iter > st->stall_iters) f |= SF_STALL | SF_DONE;
+    if (!finite(rn2)) f |= SF_NAN | SF_DONE;
+    if (rn2 <= st->stop2 || (f & (SF_DONE | SF_LIMIT))) {
+      f |= SF_DONE;
+    } else {
+      const cplx<double> ro = st->rho_old;
+      st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
+      st->rho = rho;
+      if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2) f |= SF_RESTART | SF_MR;
+      f &= ~(SF_PAUSE | SF_BREAKDOWN);
+    }
+  } else {   // FIN_BNORM
+    st->bnorm2 = tot[0];
+    const double a = st->rtol * st->rtol * tot[0], c = st->atol * st->atol;
+    st->stop2 = a > c ? a : c;
+  }
+  st->flags = f;
 }
 
 // ---- host launch helpers ----------------------------------------------------------
@@ -223,6 +441,7 @@ MX_VEC_LAUNCH(launch_update_f64, k_update<double>)
 MX_VEC_LAUNCH(launch_accum_f32, k_accum<float>)
 MX_VEC_LAUNCH(launch_accum_f64, k_accum<double>)
 MX_VEC_LAUNCH(launch_bnorm, k_bnorm)
+MX_VEC_LAUNCH(launch_save_best, k_save_best)
 #undef MX_VEC_LAUNCH
 
 hipError_t launch_pupdate_f32(const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
@@ -233,6 +452,17 @@ hipError_t launch_pupdate_f32(const OpView<float>* ops, const SysDesc* sys, cons
 hipError_t launch_pupdate_f64(const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
   if (nb <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_pupdate<double>, dim3(nb), dim3(VB), 0, s, ops, sys, bmap);
+  return hipGetLastError();
+}
+
+hipError_t launch_fin(int kind, const SysDesc* sys, const int* syslist, int nsys, hipStream_t s) {
+  if (nsys <= 0) return hipSuccess;
+  switch (kind) {
+    case FIN_APPLY: hipLaunchKernelGGL(k_fin<FIN_APPLY>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
+    case FIN_UPDATE: hipLaunchKernelGGL(k_fin<FIN_UPDATE>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
+    case FIN_RESID: hipLaunchKernelGGL(k_fin<FIN_RESID>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
+    default: hipLaunchKernelGGL(k_fin<FIN_BNORM>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
+  }
   return hipGetLastError();
 }
 

@@ -12,15 +12,20 @@ but matrix-free on the GPU with a complex-symmetric Krylov method (COCG) instead
 scipy's qmr on an assembled CSR matrix.  Because A = A^T, the adjoint system is the
 forward operator applied to conjugated data: E_adj = conj(Pl w) with A w = Pr conj(b0).
 
-matrix_solver selects the GPU method ('cocg_mixed' -- complex64 iterations with FP64
-residual replacement, the default -- or 'cocg_fp64').  A CSR-based callable, as the
+matrix_solver selects the GPU method: 'cocg_fp64' (complex128 COCG, the default) or
+'cocg_mixed' (complex64 iterations with FP64 residual replacement and FP64 solution
+accumulation; on the PML workloads measured here it needs more iterations than FP64
+COCG, because complex64 Krylov vectors lose biorthogonality once ||r|| < ~1e-3 ||b||).  A CSR-based callable, as the
 reference accepts, cannot be honoured by a matrix-free solver and raises TypeError.
 matrix_solver_opts honours scipy qmr's keys: rtol (default 1e-5), atol (default 0),
 maxiter (default 10 * 3N) and x0 (scaled initial guess); as in the reference, passing
 E_guess writes the scaled guess into matrix_solver_opts['x0'] (solvers.py:127-132).
 Convergence means ||b - A x|| <= max(rtol ||b||, atol) for the FP64 true residual of
 the scaled system.  Like the reference, a solve that does not converge still returns
-its last iterate; the status is logged and kept in `last_info`.
+an iterate -- here the best one (lowest FP64 true residual seen), where the reference
+returns qmr's last; the status is logged and kept in `last_info`.  A complex64 solve
+that stalls (near-static omega, where omega^2 eps falls below complex64 resolution of
+the curl-curl term) continues in FP64 from its best iterate.
 """
 from __future__ import annotations
 
@@ -37,13 +42,26 @@ logger = logging.getLogger(__name__)
 
 METHODS = {'cocg_mixed': _lib.MX_MIXED, 'cocg_fp64': _lib.MX_FP64}
 
+# complex64 cannot resolve the operator when omega^2 eps falls below this fraction of the
+# curl-curl scale 4/dx^2: the gradient (longitudinal) subspace of A has eigenvalues
+# -omega^2 eps, and complex64 rounding of the curl-curl term is ~1e-7 of its scale.
+NEAR_STATIC = 1e-5
+
+
+def near_static(omega: complex, dxes, epsilon) -> bool:
+    """True when complex64 arithmetic cannot represent the operator's gradient subspace."""
+    dmin = min(float(numpy.min(numpy.abs(d))) for d in dxes[0] + dxes[1])
+    cc = 4.0 / dmin ** 2
+    w2 = abs(complex(omega)) ** 2 * float(numpy.min(numpy.abs(epsilon)))
+    return w2 < NEAR_STATIC * cc
+
 #: stats of the most recent generic() call (status, iterations, residual, ...)
 last_info: dict[str, Any] = {}
 
 
 def _method(matrix_solver) -> int:
     if matrix_solver is None:
-        return _lib.MX_MIXED
+        return _lib.MX_FP64
     if isinstance(matrix_solver, str):
         if matrix_solver not in METHODS:
             raise ValueError(f'unknown GPU method {matrix_solver!r}; choose one of {sorted(METHODS)}')
@@ -74,23 +92,29 @@ def generic(omega: complex, dxes, J, epsilon, mu=None, *, pec=None, pmc=None, ad
     if 'callback' in matrix_solver_opts:
         logger.warning('matrix_solver_opts["callback"] is ignored: iterates stay on the GPU')
 
+    fp64_reason = None
+    if mode == _lib.MX_MIXED and near_static(omega, dxes, epsilon):
+        mode, fp64_reason = _lib.MX_FP64, 'near-static: omega^2 eps below complex64 resolution'
     op = DeviceOperator(omega, dxes, epsilon, mu=mu, pec=pec, pmc=pmc, device=device)
     dev = op.device
     J_d = torch.from_numpy(numpy.ascontiguousarray(J, dtype=numpy.complex128)).to(dev)
     b = op.map(_lib.MX_MAP_RHS_ADJ if adjoint else _lib.MX_MAP_RHS, J_d, coef=-1j * complex(omega))
     del J_d
 
+    # Scaled guesses cross the boundary in the reference's convention (p = sqrt(p^2),
+    # principal root); internally p is factored per axis and may differ by a sign per
+    # entry, which MX_MAP_SIGN applies (an involution).
     if E_guess is not None:
         g = torch.from_numpy(numpy.ascontiguousarray(E_guess, dtype=numpy.complex128)).to(dev)
         x = op.map(_lib.MX_MAP_RHS_ADJ if adjoint else _lib.MX_MAP_RHS, g, coef=1.0)
-        x0_host = x.cpu().numpy()
+        x0_host = op.map(_lib.MX_MAP_SIGN, x).cpu().numpy()
         # the reference stores Pl E_guess (forward) or Pr^H E_guess (adjoint)
         matrix_solver_opts['x0'] = numpy.conj(x0_host) if adjoint else x0_host
     elif matrix_solver_opts.get('x0') is not None:
         x0 = numpy.asarray(matrix_solver_opts['x0'], dtype=numpy.complex128)
         if adjoint:
             x0 = numpy.conj(x0)
-        x = torch.from_numpy(numpy.ascontiguousarray(x0)).to(dev)
+        x = op.map(_lib.MX_MAP_SIGN, torch.from_numpy(numpy.ascontiguousarray(x0)).to(dev))
     else:
         x = torch.zeros(n3, dtype=torch.complex128, device=dev)
 
@@ -102,6 +126,8 @@ def generic(omega: complex, dxes, J, epsilon, mu=None, *, pec=None, pmc=None, ad
     stats = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol, atol=atol, maxiter=maxiter)], mode=mode)[0]
     E = op.map(_lib.MX_MAP_E_ADJ if adjoint else _lib.MX_MAP_E, x)
     last_info = dict(stats, method='cocg_fp64' if mode == _lib.MX_FP64 else 'cocg_mixed', adjoint=adjoint)
+    if fp64_reason:
+        last_info['fp64_reason'] = fp64_reason
     if stats['status'] != 'converged':
         logger.warning('FDFD solve did not converge: %s', last_info)
     else:

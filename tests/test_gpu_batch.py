@@ -1,0 +1,135 @@
+"""Batched and served solves on the GPU.
+
+  * many right-hand sides on one structure (anisotropic eps) in one batch: every
+    solution meets its residual target against the oracle's CSR operator and equals
+    the single-system solve bit for bit;
+  * an uneven batch of structures (different grid sizes, PEC blocks): every solution
+    meets the physical residual target with the oracle's operator;
+  * the continuous-batching server: each result is the same alone and in the mix
+    (bit for bit), forward and adjoint requests, with admission while others iterate.
+"""
+import numpy
+import pytest
+
+from oracle import fdfd_ref as ref
+
+pytestmark = pytest.mark.gpu
+
+
+def small_aniso(shape=(24, 20, 22), nrhs=4, seed=7):
+    from meanas_amd import workloads
+    g = numpy.random.default_rng(seed)
+    omega = 2 * numpy.pi / 12.0
+    dxes = workloads.pml_dxes(shape, omega, thickness=4)
+    eps = workloads.aniso_structure(shape, g, nbox=6).ravel()
+    Js = [workloads.gaussian_sheet(shape, int(g.integers(6, shape[0] - 6)), 3.0,
+                                   workloads.random_polarisation(g)).ravel() for _ in range(nrhs)]
+    return omega, dxes, eps, Js
+
+
+def physical_residual(omega, dxes, eps, J, E, pec=None):
+    A0 = ref.e_full(omega, dxes, eps, pec=pec)
+    b0 = -1j * omega * J
+    return numpy.linalg.norm(A0 @ E - b0) / numpy.linalg.norm(b0)
+
+
+@pytest.mark.parametrize('method', ['fp64', 'mixed'])
+def test_multi_rhs_batch(method):
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    mode = _lib.MX_FP64 if method == 'fp64' else _lib.MX_MIXED
+    omega, dxes, eps, Js = small_aniso()
+    op = DeviceOperator(omega, dxes, eps)
+    bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega) for J in Js]
+    xs = [torch.zeros_like(b) for b in bs]
+    st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=1e-9) for b, x in zip(bs, xs)], mode=mode)
+    for k, (J, x) in enumerate(zip(Js, xs)):
+        assert st[k]['status'] == 'converged', st[k]
+        E = op.map(_lib.MX_MAP_E, x).cpu().numpy()
+        assert physical_residual(omega, dxes, eps, J, E) < 1e-8
+    # the first system alone gives the same bits
+    x1 = torch.zeros_like(bs[0])
+    solve_batch([op], [dict(op=0, b=bs[0], x=x1, rtol=1e-9)], mode=mode)
+    assert torch.equal(x1, xs[0])
+
+
+def test_uneven_structures_with_pec():
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    probs = workloads.uneven_batch(count=4, seed=11, edges=(26, 30, 34))
+    ops, systems = [], []
+    for k, w in enumerate(probs):
+        op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], pec=w['pec'])
+        ops.append(op)
+        b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(w['J']).cuda(), coef=-1j * w['omega'])
+        systems.append(dict(op=k, b=b, x=torch.zeros_like(b), rtol=1e-9))
+    st = solve_batch(ops, systems, mode=_lib.MX_FP64)
+    for k, w in enumerate(probs):
+        assert st[k]['status'] == 'converged', st[k]
+        E = ops[k].map(_lib.MX_MAP_E, systems[k]['x']).cpu().numpy()
+        assert physical_residual(w['omega'], w['dxes'], w['epsilon'], w['J'], E, pec=w['pec']) < 1e-8
+        # PEC entries of E stay zero (the source avoids PEC cells)
+        assert numpy.abs(E[w['pec'] != 0]).max() == 0
+
+
+def _requests():
+    from meanas_amd import workloads
+    from meanas_amd.serve import Request
+    g = numpy.random.default_rng(5)
+    out = []
+    for k, shape in enumerate([(24, 20, 22), (20, 26, 18), (28, 18, 20), (22, 22, 22)]):
+        omega = 2 * numpy.pi / 12.0
+        dxes = workloads.pml_dxes(shape, omega, thickness=4)
+        eps = workloads.aniso_structure(shape, g, nbox=5).ravel()
+        J = workloads.gaussian_sheet(shape, 8, 3.0, workloads.random_polarisation(g)).ravel()
+        out.append(Request(omega=omega, dxes=dxes, J=J, epsilon=eps, adjoint=(k % 2 == 1), rtol=1e-9,
+                           arrival=0.0, tag=k))
+    return out
+
+
+@pytest.mark.parametrize('method', ['fp64', 'mixed'])
+def test_server_alone_equals_mixed(method):
+    from meanas_amd import _lib
+    from meanas_amd.serve import Server
+    mode = _lib.MX_FP64 if method == 'fp64' else _lib.MX_MIXED
+    reqs = _requests()
+    alone = {}
+    for r in reqs:
+        srv = Server(capacity=1, mode=mode, chunk=8)
+        res = srv.run([r], realtime=False)
+        srv.close()
+        alone[r.tag] = res[0]
+    srv = Server(capacity=3, mode=mode, chunk=8)        # fewer slots than requests: admission mid-run
+    mixed = {res.tag: res for res in srv.run(reqs, realtime=False)}
+    srv.close()
+    assert set(mixed) == set(alone)
+    for r in reqs:
+        a, m = alone[r.tag], mixed[r.tag]
+        assert m.stats['status'] == 'converged', m.stats
+        assert numpy.array_equal(a.E, m.E), r.tag
+        # and the solution is right: forward vs the oracle operator, adjoint vs the
+        # conjugate-transposed system (fdfd/solvers.py:120-137)
+        if r.adjoint:
+            A0 = ref.e_full(r.omega, r.dxes, r.epsilon)
+            Pl, Pr = ref.e_full_preconditioners(r.dxes)
+            Aadj = (Pl @ A0 @ Pr).conj().T
+            z = Pl.diagonal().conj() ** -1 * m.E          # E = Pl^H z
+            bb = Pr.conj().T @ (-1j * r.omega * r.J)
+            assert numpy.linalg.norm(Aadj @ z - bb) / numpy.linalg.norm(bb) < 1e-8
+        else:
+            assert physical_residual(r.omega, r.dxes, r.epsilon, r.J, m.E) < 1e-8
+
+
+def test_server_matches_generic():
+    from meanas_amd import _lib
+    from meanas_amd.fdfd import solvers
+    from meanas_amd.serve import Server
+    r = _requests()[0]
+    srv = Server(capacity=2, mode=_lib.MX_FP64, chunk=16)
+    res = srv.run([r], realtime=False)[0]
+    srv.close()
+    E = solvers.generic(r.omega, r.dxes, r.J, r.epsilon, matrix_solver='cocg_fp64',
+                        matrix_solver_opts=dict(rtol=1e-9))
+    assert numpy.linalg.norm(res.E - E) <= 1e-7 * numpy.linalg.norm(E)

@@ -161,13 +161,23 @@ def test_batch_equals_alone_bitwise():
         assert numpy.array_equal(mixed[k], alone[i])
 
 
-def test_maxiter_returns_last_iterate_with_status():
+@pytest.mark.parametrize('maxiter', [3, 40])
+def test_maxiter_returns_best_iterate_with_status(maxiter):
+    """An unconverged solve returns its best FP64 iterate (lowest true residual seen at a
+    residual replacement) and reports that iterate's residual; the reference returns its
+    last qmr iterate, silently (fdfd/solvers.py:34-36,62)."""
     from meanas_amd.fdfd import solvers
     omega, dxes, J, eps, *_ = small_problem((6, 7, 9), 40)
-    E = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(rtol=1e-12, maxiter=3))
-    assert solvers.last_info['status'] == 'maxiter'
-    assert solvers.last_info['iterations'] == 3
-    assert numpy.isfinite(E).all() and numpy.linalg.norm(E) > 0
+    E = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(rtol=1e-14, maxiter=maxiter))
+    info = solvers.last_info
+    assert info['status'] == 'maxiter' and info['iterations'] == maxiter, info
+    assert numpy.isfinite(E).all()
+    A0 = ref.e_full(omega, dxes, eps)
+    b0 = -1j * omega * J
+    p = ref.e_full_preconditioner_vector(dxes)
+    scaled = numpy.linalg.norm(p * (A0 @ E - b0)) / numpy.linalg.norm(p * b0)
+    assert abs(scaled - info['rel_residual']) <= 1e-9 * max(1.0, scaled)
+    assert info['rel_residual'] <= 1.0
 
 
 def test_zero_source_gives_zero_field():
@@ -196,3 +206,13 @@ def test_e_guess_semantics():
     numpy.testing.assert_allclose(opts['x0'], ref.e_full_preconditioner_vector(dxes) * E, rtol=1e-12)
     assert solvers.last_info['iterations'] <= 2
     assert rel(E2, E) < 1e-9
+    # and a scaled guess handed back through matrix_solver_opts['x0'] is honoured
+    E3 = solvers.generic(omega, dxes, J, eps, matrix_solver_opts=dict(opts))
+    assert solvers.last_info['iterations'] <= 2 and rel(E3, E) < 1e-9
+    # adjoint: x0 = Pr^H E_guess (fdfd/solvers.py:128-129)
+    Ea = solvers.generic(omega, dxes, J, eps, adjoint=True, matrix_solver_opts=dict(rtol=1e-11))
+    opts_a = dict(rtol=1e-11)
+    solvers.generic(omega, dxes, J, eps, adjoint=True, matrix_solver_opts=opts_a, E_guess=Ea)
+    numpy.testing.assert_allclose(opts_a['x0'], numpy.conj(1 / ref.e_full_preconditioner_vector(dxes)) * Ea,
+                                  rtol=1e-12)
+    assert solvers.last_info['iterations'] <= 2
