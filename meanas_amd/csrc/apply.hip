@@ -25,8 +25,7 @@ __device__ __forceinline__ int wrapi(int v, int n) {
 
 template <typename T>
 __device__ __forceinline__ cplx<T> load_c(const T* base, int idx) {
-  const cplx<T>* p = reinterpret_cast<const cplx<T>*>(base);
-  return p[idx];
+  return gload<T>(base, idx);
 }
 
 // material value (eps or 1/mu) of component c at cell `cell`
@@ -34,8 +33,15 @@ template <typename T>
 __device__ __forceinline__ cplx<T> load_mat(const T* base, bool is_complex, bool aniso, int c,
                                             int cell, int ncell) {
   const int idx = aniso ? c * ncell + cell : cell;
-  if (is_complex) return load_c<T>(base, idx);
-  return cplx<T>{base[idx], T(0)};
+  if (is_complex) return gload<T>(base, idx);
+  return cplx<T>{gload1<T>(base, idx), T(0)};
+}
+
+// forward-difference coefficients (fa, fb) of entry n of a per-axis table
+template <typename T> struct FaFb { cplx<T> fa, fb; };
+template <typename T>
+__device__ __forceinline__ FaFb<T> coef_at(const AxisCoef<T>* tab, int n) {
+  return FaFb<T>{gload<T>(tab, 4 * n), gload<T>(tab, 4 * n + 1)};
 }
 
 template <typename T, int TY, int EPI, typename RT>
@@ -58,7 +64,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   SysState* st = S.st;
   bool mr = false;   // minimal-residual step: accumulate q^H p and ||q||^2 instead of p^T q
   if (EPI != EPI_PLAIN) {
-    const int f = st->flags;
+    const int f = gload1<int>(&st->flags, 0);
     if ((f & skip_flags) || (need_flags && !(f & need_flags))) return;
     mr = (f & SF_MR) != 0;
   }
@@ -108,16 +114,16 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     const int pb = ip * (int)plane;
 #pragma unroll
     for (int c = 0; c < 3; ++c) R.o[c] = load_c<T>(xin, c * nc + pb + own_off);
-    R.mo = has_mask ? op.mask[pb + own_off] : 0;
+    R.mo = has_mask ? gload1<uint8_t>(op.mask, pb + own_off) : 0;
     if (has_row) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) R.r[c] = load_c<T>(xin, c * nc + pb + row_off);
-      R.mr = has_mask ? op.mask[pb + row_off] : 0;
+      R.mr = has_mask ? gload1<uint8_t>(op.mask, pb + row_off) : 0;
     }
     if (has_col) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) R.c[c] = load_c<T>(xin, c * nc + pb + col_off);
-      R.mc = has_mask ? op.mask[pb + col_off] : 0;
+      R.mc = has_mask ? gload1<uint8_t>(op.mask, pb + col_off) : 0;
     }
   };
   auto put = [&](int buf, int lds, const cplx<T> (&v)[3], uint8_t m) {
@@ -135,10 +141,10 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   Regs RA;
 
   // per-thread y/z coefficients (wrapped indices)
-  const AxisCoef<T> cyj = cy[jw], czk = cz[kw];
-  const cplx<T> fa_ym = cy[wrapi(j - 1, ny)].fa, fa_zm = cz[wrapi(k - 1, nz)].fa;
-  const AxisCoef<T> cyh = cy[wrapi(j0 - 1, ny)];   // halo row j0-1
-  const AxisCoef<T> czh = cz[wrapi(k0 - 1, nz)];   // halo column k0-1
+  const FaFb<T> cyj = coef_at(cy, jw), czk = coef_at(cz, kw);
+  const cplx<T> fa_ym = coef_at(cy, wrapi(j - 1, ny)).fa, fa_zm = coef_at(cz, wrapi(k - 1, nz)).fa;
+  const FaFb<T> cyh = coef_at(cy, wrapi(j0 - 1, ny));   // halo row j0-1
+  const FaFb<T> czh = coef_at(cz, wrapi(k0 - 1, nz));   // halo column k0-1
 
   // H multiplier at (cell, component): pmc keep * 1/mu
   auto hscale = [&](cplx<T> h, int c, uint8_t m, int cell) -> cplx<T> {
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     fetch(RA, ib);
     commit(RA, 1);
     __syncthreads();
-    const AxisCoef<T> cxm = cx[im1];
+    const FaFb<T> cxm = coef_at(cx, im1);
     const int r = ty + 1, q = tz + 1;
     const cplx<T>* A = &sE[0][0][0][0];
     const cplx<T>* B = &sE[1][0][0][0];
@@ -191,8 +197,8 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       epsv[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
-    const AxisCoef<T> cxi = cx[i];
-    const cplx<T> fa_xm = cx[wrapi(i - 1, nx)].fa;
+    const FaFb<T> cxi = coef_at(cx, i);
+    const cplx<T> fa_xm = coef_at(cx, wrapi(i - 1, nx)).fa;
     __syncthreads();
 
     const cplx<T>* C = &sE[cb][0][0][0];
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     if (ty == TY - 1 && tz < TY) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
       const int rr = tz + 1;
       const int jj = wrapi(j0 + tz, ny);
-      const AxisCoef<T> cyr = cy[jj];
+      const FaFb<T> cyr = coef_at(cy, jj);
       const uint8_t mh = sM[cb][rr][0];
       const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
       cplx<T> hx = (cyr.fa * e_(2, rr + 1, 0) - cyr.fb * e_(2, rr, 0)) - (czh.fa * e_(1, rr, 1) - czh.fb * e_(1, rr, 0));
@@ -248,16 +254,16 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
       for (int c = 0; c < 3; ++c) {
         const int idx = c * nc + cell;
         if constexpr (EPI == EPI_RESID) {
-          const cplx<double> bb = load_c<double>(bvec, idx);
+          const cplx<double> bb = gload<double>(bvec, idx);
           const cplx<double> rr = bb - cast<double>(out[c]);
           const cplx<RT> rs = cast<RT>(rr);
-          reinterpret_cast<cplx<RT>*>(rout)[idx] = rs;
+          gstore<RT>(rout, idx, rs);
           const cplx<double> rd = cast<double>(rs);
           acc[0] += norm2(rr);
           acc[1] += rd.re * rd.re - rd.im * rd.im;
           acc[2] += 2.0 * rd.re * rd.im;
         } else {
-          reinterpret_cast<cplx<T>*>(yout)[idx] = out[c];
+          gstore<T>(yout, idx, out[c]);
           if constexpr (EPI == EPI_COCG) {
             if (!mr) {
               const cplx<T> pq = ec[c] * out[c];

@@ -43,6 +43,61 @@ __host__ __device__ __forceinline__ cplx<double> cdiv(cplx<double> a, cplx<doubl
   return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
 }
 
+// ---- global-address-space access --------------------------------------------------
+// Pointers read out of descriptor tables are generic (flat) to the compiler; flat loads
+// count against lgkmcnt, so every LDS wait would also wait for in-flight prefetches, and
+// they need 64-bit VGPR addresses.  These helpers access memory through address-space-1
+// pointers (device pass) so the compiler emits global_load/store with an SGPR base and a
+// 32-bit VGPR offset.  Complex values move as native 2-vectors.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MX_GLOBAL __attribute__((address_space(1)))
+#else
+#define MX_GLOBAL
+#endif
+template <typename T> struct Vec2 { typedef T __attribute__((ext_vector_type(2))) type; };
+template <typename T> struct Vec4 { typedef T __attribute__((ext_vector_type(4))) type; };
+
+// Byte offsets are 32-bit unsigned (the host keeps every vector below 4 GiB), which
+// lets the compiler use the SGPR-base + 32-bit VGPR-offset addressing form.
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(const void* base, int idx) {
+  typedef typename Vec2<T>::type V;
+  const unsigned off = (unsigned)idx * (unsigned)sizeof(V);
+  const V v = *(const MX_GLOBAL V*)((const MX_GLOBAL char*)base + off);
+  return cplx<T>{v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void gstore(void* base, int idx, cplx<T> a) {
+  typedef typename Vec2<T>::type V;
+  const unsigned off = (unsigned)idx * (unsigned)sizeof(V);
+  V v;
+  v.x = a.re;
+  v.y = a.im;
+  *(MX_GLOBAL V*)((MX_GLOBAL char*)base + off) = v;
+}
+template <typename T>
+__device__ __forceinline__ T gload1(const void* base, int idx) {
+  const unsigned off = (unsigned)idx * (unsigned)sizeof(T);
+  return *(const MX_GLOBAL T*)((const MX_GLOBAL char*)base + off);
+}
+
+// 16-byte pack i of a vector (2 complex64 or 1 complex128)
+template <typename T> struct PackV;
+template <> struct PackV<float> { typedef float __attribute__((ext_vector_type(4))) type; };
+template <> struct PackV<double> { typedef double __attribute__((ext_vector_type(2))) type; };
+template <typename T>
+__device__ __forceinline__ typename PackV<T>::type pload(const void* base, long long i) {
+  typedef typename PackV<T>::type V;
+  const unsigned off = (unsigned)i * 16u;
+  return *(const MX_GLOBAL V*)((const MX_GLOBAL char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void pstore(void* base, long long i, typename PackV<T>::type v) {
+  typedef typename PackV<T>::type V;
+  const unsigned off = (unsigned)i * 16u;
+  *(MX_GLOBAL V*)((MX_GLOBAL char*)base + off) = v;
+}
+
 // Per-axis coefficients at index n: fa multiplies f[n+1], fb multiplies f[n] in the
 // scaled forward difference; se = sqrt(dx_e[n]), sh = sqrt(dx_h[n]) give the diagonal
 // scaling p used to map J -> b and x -> E.
@@ -173,7 +228,7 @@ template <int K>
 __device__ __forceinline__ void store_partials(const double (&v)[K], double* partial, int local, int nblk) {
   if (threadIdx.x == 0 && threadIdx.y == 0) {
 #pragma unroll
-    for (int q = 0; q < K; ++q) partial[q * nblk + local] = v[q];
+    for (int q = 0; q < K; ++q) ((MX_GLOBAL double*)partial)[q * nblk + local] = v[q];
   }
 }
 
@@ -191,7 +246,7 @@ __device__ __forceinline__ void final_sum(double (&out)[K], const double* partia
 #pragma unroll
     for (int l = 0; l < kFinLoads; ++l) {
       const int b = tid + l * nthreads;
-      v[q][l] = b < nblk ? partial[q * nblk + b] : 0.0;
+      v[q][l] = b < nblk ? gload1<double>(partial, q * nblk + b) : 0.0;
     }
 #pragma unroll
   for (int q = 0; q < K; ++q) {

@@ -66,7 +66,8 @@ long long ncell_of(const mx_operator& op) { return (long long)op.nx * op.ny * op
 
 bool valid_op(const mx_operator& op, int mode, std::string* why) {
   if (op.nx < 1 || op.ny < 1 || op.nz < 1) { *why = "grid dimensions must be >= 1"; return false; }
-  if (ncell_of(op) > (1LL << 31) / 3) { *why = "grid too large (3N must fit in int32 indexing of a plane)"; return false; }
+  // every complex128 vector must stay below 4 GiB: kernels address it with 32-bit byte offsets
+  if (ncell_of(op) * 3 * 16 >= (1LL << 32)) { *why = "grid too large (a complex128 field must be < 4 GiB, i.e. < 89M cells)"; return false; }
   const bool need32 = (mode == MX_MIXED);
   if (!op.coef64 || !op.eps64) { *why = "coef64/eps64 are required"; return false; }
   if (need32 && (!op.coef32 || !op.eps32)) { *why = "coef32/eps32 are required for the mixed mode"; return false; }

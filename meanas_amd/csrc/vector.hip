@@ -26,11 +26,34 @@ __device__ __forceinline__ bool skip_sys(int f, int skip, int need) {
   return (f & skip) || (need && !(f & need));
 }
 
-// 16-byte pack of complex values: 2 complex64 or 1 complex128 (one dwordx4 access).
+// 16-byte pack of complex values: 2 complex64 or 1 complex128 (one dwordx4 access),
+// loaded and stored through global-address-space pointers (see common.hpp).
 template <typename T> struct alignas(16) Pack {
   static constexpr int K = 16 / sizeof(cplx<T>);
   cplx<T> v[K];
 };
+template <typename T>
+__device__ __forceinline__ Pack<T> ld(const void* base, long long i) {
+  const typename PackV<T>::type w = pload<T>(base, i);
+  Pack<T> p;
+  if constexpr (Pack<T>::K == 2) {
+    p.v[0] = cplx<T>{w[0], w[1]};
+    p.v[1] = cplx<T>{w[2], w[3]};
+  } else {
+    p.v[0] = cplx<T>{w[0], w[1]};
+  }
+  return p;
+}
+template <typename T>
+__device__ __forceinline__ void stp(void* base, long long i, const Pack<T>& p) {
+  typename PackV<T>::type w;
+  if constexpr (Pack<T>::K == 2) {
+    w[0] = p.v[0].re; w[1] = p.v[0].im; w[2] = p.v[1].re; w[3] = p.v[1].im;
+  } else {
+    w[0] = p.v[0].re; w[1] = p.v[0].im;
+  }
+  pstore<T>(base, i, w);
+}
 
 // Contiguous pack range [*b, *e) of this block: the vector is split into nblk equal runs.
 __device__ __forceinline__ void block_range(long long npk, int local, int nblk, long long* b, long long* e) {
@@ -54,7 +77,7 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
                                               const BlockMap* __restrict__ bmap) {
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
-  const int f = S.st->flags;
+  const int f = gload1<int>(&S.st->flags, 0);
   if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
   const cplx<T>* __restrict__ r = reinterpret_cast<const cplx<T>*>(S.vec[VS_R]);
   cplx<T>* __restrict__ p = reinterpret_cast<cplx<T>*>(S.vec[VS_P]);
@@ -70,20 +93,18 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
     }
     return;
   }
-  const Pack<T>* __restrict__ r4 = reinterpret_cast<const Pack<T>*>(r);
-  Pack<T>* __restrict__ p4 = reinterpret_cast<Pack<T>*>(p);
   constexpr int K = Pack<T>::K;
   long long pb, pe;
   block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
   const bool restart = (f & SF_RESTART) != 0;
-  const cplx<T> beta = restart ? zero<T>() : cast<T>(S.st->beta);
+  const cplx<T> beta = restart ? zero<T>() : cast<T>(gload<double>(&S.st->beta, 0));
   for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
     Pack<T> a[VU], c[VU];
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
-        a[u] = r4[t.at(u)];
-        if (!restart) c[u] = p4[t.at(u)];
+        a[u] = ld<T>(r, t.at(u));
+        if (!restart) c[u] = ld<T>(p, t.at(u));
       }
 #pragma unroll
     for (int u = 0; u < VU; ++u)
@@ -92,7 +113,7 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
 #pragma unroll
           for (int k = 0; k < K; ++k) a[u].v[k] = a[u].v[k] + beta * c[u].v[k];
         }
-        p4[t.at(u)] = a[u];
+        stp<T>(p, t.at(u), a[u]);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0)
@@ -106,16 +127,12 @@ __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ syste
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   SysState* st = S.st;
-  if (skip_sys(st->flags, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  if (skip_sys(gload1<int>(&st->flags, 0), SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
   cplx<T>* __restrict__ x = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   cplx<T>* __restrict__ r = reinterpret_cast<cplx<T>*>(S.vec[VS_R]);
   const cplx<T>* __restrict__ p = reinterpret_cast<const cplx<T>*>(S.vec[VS_P]);
   const cplx<T>* __restrict__ q = reinterpret_cast<const cplx<T>*>(S.vec[VS_Q]);
-  Pack<T>* __restrict__ x4 = reinterpret_cast<Pack<T>*>(x);
-  Pack<T>* __restrict__ r4 = reinterpret_cast<Pack<T>*>(r);
-  const Pack<T>* __restrict__ p4 = reinterpret_cast<const Pack<T>*>(p);
-  const Pack<T>* __restrict__ q4 = reinterpret_cast<const Pack<T>*>(q);
-  const cplx<T> alpha = cast<T>(st->alpha);
+  const cplx<T> alpha = cast<T>(gload<double>(&st->alpha, 0));
   const long long n = S.n3;
   T rr_re = 0, rr_im = 0, nr = 0;
   auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T> pv, cplx<T> qv) {
@@ -133,18 +150,18 @@ __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ syste
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
-        xv[u] = x4[t.at(u)];
-        rv[u] = r4[t.at(u)];
-        pv[u] = p4[t.at(u)];
-        qv[u] = q4[t.at(u)];
+        xv[u] = ld<T>(x, t.at(u));
+        rv[u] = ld<T>(r, t.at(u));
+        pv[u] = ld<T>(p, t.at(u));
+        qv[u] = ld<T>(q, t.at(u));
       }
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
 #pragma unroll
         for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
-        x4[t.at(u)] = xv[u];
-        r4[t.at(u)] = rv[u];
+        stp<T>(x, t.at(u), xv[u]);
+        stp<T>(r, t.at(u), rv[u]);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
@@ -167,7 +184,6 @@ __global__ __launch_bounds__(VB) void k_accum(const SysDesc* __restrict__ system
   cplx<double>* __restrict__ x = reinterpret_cast<cplx<double>*>(S.vec[VS_X]);
   cplx<T>* __restrict__ xl = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   const long long n = S.n3;
-  Pack<T>* __restrict__ xl4 = reinterpret_cast<Pack<T>*>(xl);
   constexpr int K = Pack<T>::K;
   long long pb, pe;
   block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
@@ -180,16 +196,16 @@ __global__ __launch_bounds__(VB) void k_accum(const SysDesc* __restrict__ system
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
-        a[u] = xl4[t.at(u)];
+        a[u] = ld<T>(xl, t.at(u));
 #pragma unroll
-        for (int k = 0; k < K; ++k) xv[u][k] = x[t.at(u) * K + k];
+        for (int k = 0; k < K; ++k) xv[u][k] = gload<double>(x, t.at(u) * K + k);
       }
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) x[t.at(u) * K + k] = xv[u][k] + cast<double>(a[u].v[k]);
-        xl4[t.at(u)] = z;
+        for (int k = 0; k < K; ++k) gstore<double>(x, t.at(u) * K + k, xv[u][k] + cast<double>(a[u].v[k]));
+        stp<T>(xl, t.at(u), z);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
@@ -212,10 +228,10 @@ __global__ __launch_bounds__(VB) void k_save_best(const SysDesc* __restrict__ sy
     cplx<double> v[VU];
 #pragma unroll
     for (int u = 0; u < VU; ++u)
-      if (t.ok(u)) v[u] = x[t.at(u)];
+      if (t.ok(u)) v[u] = gload<double>(x, t.at(u));
 #pragma unroll
     for (int u = 0; u < VU; ++u)
-      if (t.ok(u)) xb[t.at(u)] = v[u];
+      if (t.ok(u)) gstore<double>(xb, t.at(u), v[u]);
   }
 }
 
@@ -231,7 +247,7 @@ __global__ __launch_bounds__(VB) void k_bnorm(const SysDesc* __restrict__ system
   double acc[1] = {0.0};
   long long pb, pe;
   block_range(n, bm.local, S.nblk_vec, &pb, &pe);
-  for (long long e = pb + threadIdx.x; e < pe; e += VB) acc[0] += norm2(b[e]);
+  for (long long e = pb + threadIdx.x; e < pe; e += VB) acc[0] += norm2(gload<double>(b, e));
   block_sum<1>(acc, s_red);
   store_partials<1>(acc, S.partial, bm.local, S.nblk_vec);
 }
@@ -244,14 +260,13 @@ __global__ __launch_bounds__(VB) void k_zero_slot(const SysDesc* __restrict__ sy
   if (skip_sys(S.st->flags, SF_DONE | SF_INACTIVE, 0)) return;
   cplx<T>* __restrict__ v = reinterpret_cast<cplx<T>*>(S.vec[slot]);
   const long long n = S.n3;
-  Pack<T>* __restrict__ v4 = reinterpret_cast<Pack<T>*>(v);
   constexpr int K = Pack<T>::K;
   long long pb, pe;
   block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
   Pack<T> z;
 #pragma unroll
   for (int k = 0; k < K; ++k) z.v[k] = zero<T>();
-  for (long long i = pb + threadIdx.x; i < pe; i += VB) v4[i] = z;
+  for (long long i = pb + threadIdx.x; i < pe; i += VB) stp<T>(v, i, z);
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) v[n - 1] = zero<T>();
 }
 
