@@ -50,7 +50,13 @@ def parse():
     ap.add_argument('--apply-only', action='store_true',
                     help='only run the operator apply (complex64 and complex128, default tiling): '
                          'the program the hardware-counter profiles are taken on')
-    ap.add_argument('--mode', choices=['mixed', 'fp64'], default='mixed',
+    ap.add_argument('--workload', choices=['single', 'multi_rhs', 'uneven', 'serve'], default='single',
+                    help='single: the 128^3 solve (the metric); multi_rhs: 16 right-hand sides on one '
+                         '128x96x96 anisotropic structure; uneven: 32 structures of uneven size with PEC; '
+                         'serve: Poisson request stream with continuous batching')
+    ap.add_argument('--count', type=int, default=0, help='structures (uneven) or requests (serve)')
+    ap.add_argument('--qps', type=float, default=4.0, help='serve: arrival rate')
+    ap.add_argument('--mode', choices=['mixed', 'fp64'], default='fp64',
                     help='solver of the timed step (the other one is the same-GPU comparison)')
     return ap.parse_args()
 
@@ -123,6 +129,82 @@ def apply_sweep(op, dev):
     print(json.dumps(dict(copy_bytes=2 * nbytes, us=round(us, 2), gbs=round(2 * nbytes / (us * 1e-6) / 1e9, 1))))
 
 
+def batch_workloads(args):
+    """Multi-RHS, uneven multi-structure and serving workloads (one JSON line each)."""
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    dev = torch.device('cuda:0')
+    mode = _lib.MX_FP64 if args.mode == 'fp64' else _lib.MX_MIXED
+    if args.workload == 'multi_rhs':
+        w = workloads.aniso_multi()
+        op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
+        bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).to(dev), coef=-1j * w['omega']) for J in w['Js']]
+        xs = [torch.zeros_like(b) for b in bs]
+        systems = [dict(op=0, b=b, x=x, rtol=args.rtol, maxiter=200000) for b, x in zip(bs, xs)]
+        solve_batch([op], systems[:1], mode=mode)            # warm-up (autotune, code load)
+        for x in xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solve_batch([op], systems, mode=mode)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        line = dict(metric='solves/s, 16 right-hand sides on one structure', value=len(xs) / wall, unit='solves/s',
+                    wall_s=wall, converged=sum(s['status'] == 'converged' for s in st),
+                    iterations=[s['iterations'] for s in st],
+                    config={'workload': 'aniso_multi: 128x96x96, anisotropic box eps, 16 Gaussian sheets', 'rtol': args.rtol,
+                            'mode': args.mode})
+    elif args.workload == 'uneven':
+        probs = workloads.uneven_batch(count=args.count or 32)
+        ops, systems = [], []
+        for k, w in enumerate(probs):
+            op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], pec=w['pec'], device=dev)
+            ops.append(op)
+            b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(w['J']).to(dev), coef=-1j * w['omega'])
+            systems.append(dict(op=k, b=b, x=torch.zeros_like(b), rtol=args.rtol, maxiter=200000))
+        for op in ops:                                       # autotune outside the timed region
+            from meanas_amd.fdfd.autotune import tuned_slab
+            tuned_slab(op, mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solve_batch(ops, systems, mode=mode)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cells = sum(op.ncell for op in ops)
+        line = dict(metric='solves/s, uneven batch of structures in one launch sequence', value=len(ops) / wall,
+                    unit='solves/s', wall_s=wall, converged=sum(s['status'] == 'converged' for s in st),
+                    total_cells=cells, iterations_max=max(s['iterations'] for s in st),
+                    systems=[dict(shape=op.shape, omega=round(w['omega'], 5), status=x['status'], it=x['iterations'],
+                                  res=x['rel_residual']) for op, w, x in zip(ops, probs, st)],
+                    config={'workload': f'uneven_batch: {len(ops)} structures, edges 48-128, 5% PEC', 'rtol': args.rtol,
+                            'mode': args.mode})
+    else:
+        from meanas_amd.serve import Request, Server, summarize
+        reqs = []
+        for k, r in enumerate(workloads.serving_stream(args.count or 40, args.qps)):
+            reqs.append(Request(omega=r['omega'], dxes=r['dxes'], J=r['J'], epsilon=r['epsilon'], adjoint=r['adjoint'],
+                                rtol=args.rtol, arrival=r['arrival'], tag=dict(k=k, adjoint=r['adjoint'], shape=r['shape'])))
+        srv = Server(capacity=16, mode=mode, chunk=16)
+        # warm the autotuner cache for every shape outside the timed run
+        from meanas_amd.fdfd.autotune import tuned_slab
+        for r in reqs:
+            tuned_slab(srv.prepare(r).op, mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.run(reqs, realtime=True)
+        wall = time.perf_counter() - t0
+        srv.close()
+        summ = summarize(res, wall)
+        line = dict(metric='served solves/s with p50/p99 latency (continuous batching)', value=summ['solves_per_s'],
+                    unit='solves/s', latency_ms={'p50': summ['p50_ms'], 'p99': summ['p99_ms']}, **summ,
+                    config={'workload': f'serving: {len(reqs)} Poisson requests at {args.qps} QPS, edges lognormal '
+                                        'in [48,256], 30% adjoint, anisotropic box eps', 'rtol': args.rtol,
+                            'mode': args.mode, 'capacity': 16})
+    line.update(n_gpus=1, dtype='c128' if args.mode == 'fp64' else 'c64 inner + f64 residual', data='synthetic')
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     world = int(os.environ.get('WORLD_SIZE', '1'))
@@ -136,6 +218,9 @@ def main():
     from meanas_amd.fdfd.device import DeviceOperator, solve_batch
 
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    if args.workload != 'single':
+        batch_workloads(args)
+        return
     dev = torch.device('cuda:0')
     w = workloads.boxes_128(args.size)
     log(f'workload {w["name"]} shape {w["shape"]}')
@@ -153,13 +238,14 @@ def main():
             print(json.dumps(dict(mode=mode, us=round(us, 2), bytes_per_cell=bpc,
                                   gbs=round(bpc * op.ncell / (us * 1e-6) / 1e9, 1))), flush=True)
         # a plain copy of known size: reference for scaling the FETCH_SIZE / WRITE_SIZE counters
-        nb = 64 << 20
+        # (1 GiB each way, far larger than the L2 and the last-level cache, so its reads miss)
+        nb = 1 << 30
         a = torch.ones(nb // 4, dtype=torch.float32, device=dev)
         c = torch.empty_like(a)
-        for _ in range(args.apply_reps):
+        for _ in range(3):
             c.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps(dict(copy_bytes_each_way=nb, reps=args.apply_reps)), flush=True)
+        print(json.dumps(dict(copy_bytes_each_way=nb, reps=3)), flush=True)
         return
     J = torch.from_numpy(w['J']).to(dev)
     b = op.map(_lib.MX_MAP_RHS, J, coef=-1j * w['omega'])
@@ -190,27 +276,31 @@ def main():
         log(f'step {i}: {lat[-1] * 1e3:.1f} ms {st}')
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    E_mixed = op.map(_lib.MX_MAP_E, x)
+    E_step = op.map(_lib.MX_MAP_E, x)
     ok = all(s['status'] == 'converged' and s['rel_residual'] <= args.rtol for s in stats)
 
-    # ---- same-GPU FP64 COCG and the field-error reference
+    # ---- same-GPU comparison in the other precision mode, and the field-error reference
     extra = {}
     if not args.no_fp64:
-        x64 = torch.zeros_like(b)
+        other = _lib.MX_MIXED if step_mode == _lib.MX_FP64 else _lib.MX_FP64
+        x2 = torch.zeros_like(b)
         t0 = time.perf_counter()
-        st64 = solve(_lib.MX_FP64, args.rtol, x64)
+        st2 = solve(other, args.rtol, x2)
         torch.cuda.synchronize()
-        t64 = time.perf_counter() - t0
-        log(f'fp64 cocg: {t64 * 1e3:.1f} ms {st64}')
-        st_ref = solve(_lib.MX_FP64, 1e-11, x64)
-        E_ref = op.map(_lib.MX_MAP_E, x64)
-        ferr = float(torch.linalg.vector_norm(E_mixed - E_ref) / torch.linalg.vector_norm(E_ref))
-        log(f'fp64 reference solve (1e-11): {st_ref}; field error of mixed solve {ferr:.3e}')
-        extra.update({'fp64_cocg': dict(solves_per_s=1.0 / t64, ms=t64 * 1e3, iterations=st64['iterations'],
-                                        rel_residual=st64['rel_residual']),
-                      'mixed_over_fp64_speedup': (1.0 / numpy.median(lat)) / (1.0 / t64),
+        t2 = time.perf_counter() - t0
+        log(f'comparison ({"mixed" if other == _lib.MX_MIXED else "fp64"}): {t2 * 1e3:.1f} ms {st2}')
+        st_ref = solve(_lib.MX_FP64, 1e-11, x2)
+        E_ref = op.map(_lib.MX_MAP_E, x2)
+        ferr = float(torch.linalg.vector_norm(E_step - E_ref) / torch.linalg.vector_norm(E_ref))
+        log(f'fp64 reference solve (1e-11): {st_ref}; field error of the measured solve {ferr:.3e}')
+        t_step = float(numpy.median(lat))
+        t_fp64, t_mixed = (t_step, t2) if step_mode == _lib.MX_FP64 else (t2, t_step)
+        extra.update({'comparison': dict(mode='mixed' if other == _lib.MX_MIXED else 'fp64', solves_per_s=1.0 / t2,
+                                         ms=t2 * 1e3, iterations=st2['iterations'], rel_residual=st2['rel_residual'],
+                                         status=st2['status'], fp64_fallback=bool(st2.get('fp64_fallback', False))),
+                      'mixed_over_fp64_speedup': t_fp64 / t_mixed,
                       'field_error_vs_fp64_rtol_1e-11': ferr})
-        del x64
+        del x2
 
     # ---- apply kernel roofline: HIP events on the stream the kernel is launched on
     g = torch.Generator(device=dev).manual_seed(0)
@@ -271,7 +361,8 @@ def main():
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': ms,
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'c64 inner + f64 residual/accumulation',
+        'dtype': 'c128' if step_mode == _lib.MX_FP64 else 'c64 inner + f64 residual/accumulation',
+        'method': 'cocg_fp64' if step_mode == _lib.MX_FP64 else 'cocg_mixed',
         'data': 'synthetic (seeded box dielectrics, Gaussian sheet source)',
         'config': {'workload': f'{w["name"]}: {w["shape"][0]}x{w["shape"][1]}x{w["shape"][2]} Yee grid, '
                                '10-cell SC-PML, 40 dielectric boxes, Gaussian Jy sheet, rtol 1e-8',

@@ -76,6 +76,8 @@ typedef struct mx_operator {
 typedef struct mx_system {
   int32_t op;        /* index into the operator array of the call */
   int32_t maxiter;   /* inner iteration limit (reference default: 10 * 3N, scipy qmr) */
+  int32_t slab;      /* x-planes per apply block for this system (0 = mx_options.slab / automatic) */
+  int32_t pad;
   double rtol, atol; /* stop when ||b - A x|| <= max(rtol ||b||, atol) (scaled system) */
   const void* b;     /* complex128 3N, scaled right-hand side */
   void* x;           /* complex128 3N: in = initial guess (scaled), out = solution (scaled) */
@@ -90,10 +92,10 @@ typedef struct mx_options {
   int32_t check_every; /* iterations between residual-replacement checks (0 = automatic) */
   int32_t use_graph;   /* 1: replay hipGraph-captured iteration chunks, 0: plain launches */
   double delta;        /* replacement trigger: recursive ||r|| < delta * max ||r|| (0 = automatic) */
-  int32_t stall_iters; /* stall window: an FP64 residual replacement is forced after this many
-                          iterations without one, and the solve stops with MX_STALLED when its best
-                          true residual has not halved within this many (0 = automatic: 2000;
-                          < 0 = never) */
+  int32_t stall_iters; /* stall window: the solve stops with MX_STALLED when its best true residual
+                          has not halved within this many iterations (0 = automatic: 2000 for
+                          MX_MIXED, never for MX_FP64; < 0 = never).  An FP64 residual replacement
+                          is forced every min(stall_iters, 2000) iterations without one. */
   int32_t stall_limit; /* stop with MX_STALLED after this many consecutive replacements at which the
                           recursive residual had drifted >= 100x below the true one (0 = automatic: 3;
                           < 0 = never) */

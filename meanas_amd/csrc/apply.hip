@@ -12,6 +12,9 @@
 // computes H on the tile plus a one-cell halo on the minus side into LDS, and forms the
 // output from H(i), its y-/z- neighbours in LDS and H(i-1) kept in registers.  z is the
 // contiguous axis: a wave reads 64 consecutive complex values per row.
+#include <cstdlib>
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace mx {
@@ -37,6 +40,15 @@ __device__ __forceinline__ cplx<T> load_mat(const T* base, bool is_complex, bool
   return cplx<T>{gload1<T>(base, idx), T(0)};
 }
 
+// fa x1 - fb x0.  REAL: both coefficients are real (the axis is outside the PML here),
+// which halves the multiplications.
+template <bool REAL, typename T>
+__device__ __forceinline__ cplx<T> dif(const cplx<T>& fa, const cplx<T>& x1, const cplx<T>& fb, const cplx<T>& x0) {
+  if constexpr (REAL) return cplx<T>{fa.re * x1.re - fb.re * x0.re, fa.re * x1.im - fb.re * x0.im};
+  else return fa * x1 - fb * x0;
+}
+template <typename T> __device__ __forceinline__ bool is_real(const cplx<T>& a) { return a.im == T(0); }
+
 // forward-difference coefficients (fa, fb) of entry n of a per-axis table
 template <typename T> struct FaFb { cplx<T> fa, fb; };
 template <typename T>
@@ -44,8 +56,13 @@ __device__ __forceinline__ FaFb<T> coef_at(const AxisCoef<T>* tab, int n) {
   return FaFb<T>{gload<T>(tab, 4 * n), gload<T>(tab, 4 * n + 1)};
 }
 
-template <typename T, int TY, int EPI, typename RT>
-__global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__ ops,
+// Occupancy target: 4 waves per SIMD for complex64 (<= 128 VGPRs); complex128 needs more
+// than 168 VGPRs, so forcing 3 waves would spill.
+template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : 2;
+
+// Prefetch depth (planes of loads in flight beyond the one being committed).
+template <typename T, int TY, int EPI, typename RT, int PD>
+__global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) / TY : 1) void k_apply(const OpView<T>* __restrict__ ops,
                                                    const SysDesc* __restrict__ systems,
                                                    const BlockMap* __restrict__ bmap,
                                                    int in_slot, int out_slot,
@@ -137,8 +154,9 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     if (has_row) put(buf, row_lds, R.r, R.mr);
     if (has_col) put(buf, col_lds, R.c, R.mc);
   };
-  // Register prefetch: the loads of plane i+2 are in flight while plane i is computed.
-  Regs RA;
+  // Register prefetch: the loads of planes i+2 .. i+1+PD are in flight while plane i is
+  // computed (PD register sets, used round-robin).
+  Regs RA, RB;
 
   // per-thread y/z coefficients (wrapped indices)
   const FaFb<T> cyj = coef_at(cy, jw), czk = coef_at(cz, kw);
@@ -178,11 +196,14 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     hpy = hscale(hy, 1, m, cell);
     hpz = hscale(hz, 2, m, cell);
     fetch(RA, wrapi(ib + 1, nx));                 // plane ib+1, committed by the first step
+    if constexpr (PD == 2)
+      if (ib + 1 < ie) fetch(RB, wrapi(ib + 2, nx));   // plane ib+2, committed by the second
     __syncthreads();
   }
 
   double acc[3] = {0.0, 0.0, 0.0};
   const cplx<T> w2 = op.w2;
+  const bool diag_real = !(op.flags & OPF_EPS_COMPLEX) && w2.im == T(0);
   int cb = 1;
   T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
@@ -191,7 +212,7 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
   auto step = [&](int i, Regs& R) {
     const int nb = cb ^ 1;
     commit(R, nb);                                // plane i+1
-    if (i + 1 < ie) fetch(R, wrapi(i + 2, nx));   // prefetch plane i+2
+    if (i + PD < ie) fetch(R, wrapi(i + 1 + PD, nx));   // prefetch plane i+1+PD into the freed set
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
@@ -208,46 +229,77 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     const int r = ty + 1, q = tz + 1;
     cplx<T> ec[3] = {e_(0, r, q), e_(1, r, q), e_(2, r, q)};
     const uint8_t mown = sM[cb][r][q];
-    {
-      cplx<T> hx = (cyj.fa * e_(2, r + 1, q) - cyj.fb * ec[2]) - (czk.fa * e_(1, r, q + 1) - czk.fb * ec[1]);
-      cplx<T> hy = (czk.fa * e_(0, r, q + 1) - czk.fb * ec[0]) - (cxi.fa * n_(2, r, q) - cxi.fb * ec[2]);
-      cplx<T> hz = (cxi.fa * n_(1, r, q) - cxi.fb * ec[1]) - (cyj.fa * e_(0, r + 1, q) - cyj.fb * ec[0]);
+    // Real-coefficient fast paths: x coefficients are uniform over the plane, y
+    // coefficients over a wave (one wave is one y row); z varies per lane and stays complex.
+    const bool rx = is_real(cxi.fa) && is_real(cxi.fb) && is_real(fa_xm);
+    const bool ry = __builtin_amdgcn_readfirstlane(
+        is_real(cyj.fa) && is_real(cyj.fb) && is_real(fa_ym) && (ty != 0 || (is_real(cyh.fa) && is_real(cyh.fb))));
+    auto h_phase = [&](auto RXt, auto RYt) {
+      constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
+      const cplx<T> hx = dif<RY>(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif<false>(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
+      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, n_(2, r, q), cxi.fb, ec[2]);
+      const cplx<T> hz = dif<RX>(cxi.fa, n_(1, r, q), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
       sH[0][r][q] = hscale(hx, 0, mown, cell);
       sH[1][r][q] = hscale(hy, 1, mown, cell);
       sH[2][r][q] = hscale(hz, 2, mown, cell);
-    }
-    if (ty == 0) {   // halo row j0-1 (LDS row 0): Hx and Hz are consumed
-      const uint8_t mh = sM[cb][0][q];
-      const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
-      cplx<T> hx = (cyh.fa * e_(2, 1, q) - cyh.fb * e_(2, 0, q)) - (czk.fa * e_(1, 0, q + 1) - czk.fb * e_(1, 0, q));
-      cplx<T> hz = (cxi.fa * n_(1, 0, q) - cxi.fb * e_(1, 0, q)) - (cyh.fa * e_(0, 1, q) - cyh.fb * e_(0, 0, q));
-      sH[0][0][q] = hscale(hx, 0, mh, hcell);
-      sH[2][0][q] = hscale(hz, 2, mh, hcell);
-    }
-    if (ty == TY - 1 && tz < TY) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
-      const int rr = tz + 1;
-      const int jj = wrapi(j0 + tz, ny);
-      const FaFb<T> cyr = coef_at(cy, jj);
-      const uint8_t mh = sM[cb][rr][0];
-      const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
-      cplx<T> hx = (cyr.fa * e_(2, rr + 1, 0) - cyr.fb * e_(2, rr, 0)) - (czh.fa * e_(1, rr, 1) - czh.fb * e_(1, rr, 0));
-      cplx<T> hy = (czh.fa * e_(0, rr, 1) - czh.fb * e_(0, rr, 0)) - (cxi.fa * n_(2, rr, 0) - cxi.fb * e_(2, rr, 0));
-      sH[0][rr][0] = hscale(hx, 0, mh, hcell);
-      sH[1][rr][0] = hscale(hy, 1, mh, hcell);
+      if (ty == 0) {   // halo row j0-1 (LDS row 0): Hx and Hz are consumed
+        const uint8_t mh = sM[cb][0][q];
+        const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
+        const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
+        const cplx<T> gz = dif<RX>(cxi.fa, n_(1, 0, q), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
+        sH[0][0][q] = hscale(gx, 0, mh, hcell);
+        sH[2][0][q] = hscale(gz, 2, mh, hcell);
+      }
+      if (ty == TY - 1 && tz < TY) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
+        const int rr = tz + 1;
+        const int jj = wrapi(j0 + tz, ny);
+        const FaFb<T> cyr = coef_at(cy, jj);
+        const uint8_t mh = sM[cb][rr][0];
+        const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
+        const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
+        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, n_(2, rr, 0), cxi.fb, e_(2, rr, 0));
+        sH[0][rr][0] = hscale(gx, 0, mh, hcell);
+        sH[1][rr][0] = hscale(gy, 1, mh, hcell);
+      }
+    };
+    using T_ = std::integral_constant<bool, true>;
+    using F_ = std::integral_constant<bool, false>;
+    if (rx) {
+      if (ry) h_phase(T_{}, T_{}); else h_phase(T_{}, F_{});
+    } else {
+      if (ry) h_phase(F_{}, T_{}); else h_phase(F_{}, F_{});
     }
     __syncthreads();
 
     const cplx<T> hx0 = sH[0][r][q], hy0 = sH[1][r][q], hz0 = sH[2][r][q];
-    const cplx<T> yx = (cyj.fb * hz0 - fa_ym * sH[2][r - 1][q]) - (czk.fb * hy0 - fa_zm * sH[1][r][q - 1]);
-    const cplx<T> yy = (czk.fb * hx0 - fa_zm * sH[0][r][q - 1]) - (cxi.fb * hz0 - fa_xm * hpz);
-    const cplx<T> yz = (cxi.fb * hy0 - fa_xm * hpy) - (cyj.fb * hx0 - fa_ym * sH[0][r - 1][q]);
+    cplx<T> yx, yy, yz;
+    auto y_phase = [&](auto RXt, auto RYt) {
+      constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
+      yx = dif<RY>(cyj.fb, hz0, fa_ym, sH[2][r - 1][q]) - dif<false>(czk.fb, hy0, fa_zm, sH[1][r][q - 1]);
+      yy = dif<false>(czk.fb, hx0, fa_zm, sH[0][r][q - 1]) - dif<RX>(cxi.fb, hz0, fa_xm, hpz);
+      yz = dif<RX>(cxi.fb, hy0, fa_xm, hpy) - dif<RY>(cyj.fb, hx0, fa_ym, sH[0][r - 1][q]);
+    };
+    if (rx) {
+      if (ry) y_phase(T_{}, T_{}); else y_phase(T_{}, F_{});
+    } else {
+      if (ry) y_phase(F_{}, T_{}); else y_phase(F_{}, F_{});
+    }
     hpy = hy0;
     hpz = hz0;
     cplx<T> out[3] = {yx, yy, yz};
+    if (diag_real) {   // omega^2 eps real: one real scale of E
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      out[c] = out[c] - w2 * (epsv[c] * ec[c]);
-      if ((mown >> c) & 1) out[c] = zero<T>();
+      for (int c = 0; c < 3; ++c) {
+        const T we = w2.re * epsv[c].re;
+        out[c] = cplx<T>{out[c].re - we * ec[c].re, out[c].im - we * ec[c].im};
+        if ((mown >> c) & 1) out[c] = zero<T>();
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        out[c] = out[c] - w2 * (epsv[c] * ec[c]);
+        if ((mown >> c) & 1) out[c] = zero<T>();
+      }
     }
     if (valid) {
 #pragma unroll
@@ -281,7 +333,14 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
     }
     cb = nb;
   };
-  for (int i = ib; i < ie; ++i) step(i, RA);
+  if constexpr (PD == 1) {
+    for (int i = ib; i < ie; ++i) step(i, RA);
+  } else {
+    for (int i = ib; i < ie; i += 2) {
+      step(i, RA);
+      if (i + 1 < ie) step(i + 1, RB);
+    }
+  }
 
   if constexpr (EPI != EPI_PLAIN) {
     block_sum<3>(acc, s_red);
@@ -290,14 +349,29 @@ __global__ __launch_bounds__(64 * TY) void k_apply(const OpView<T>* __restrict__
 }
 
 // ---- host-side launch helpers ------------------------------------------------------
+// Prefetch depth per element size; MEANAS_AMD_PREFETCH=1|2 overrides it (tuning runs).
+static int prefetch_depth(size_t elem) {
+  static int forced = [] {
+    const char* v = getenv("MEANAS_AMD_PREFETCH");
+    return v ? atoi(v) : 0;
+  }();
+  if (forced == 1 || forced == 2) return forced;
+  (void)elem;
+  return 1;   // depth 2 costs the occupancy it would buy back (measured: no gain)
+}
+
 template <typename T, int EPI, typename RT>
 static hipError_t launch_apply_ty(int ty, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
                                   const void* in_ptr = nullptr, void* out_ptr = nullptr) {
-  switch (ty) {
-    case 2: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 4: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 8: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+  const int pd = prefetch_depth(sizeof(T));
+  switch (ty * 10 + pd) {
+    case 21: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT, 1>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 41: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT, 1>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 81: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT, 1>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 22: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT, 2>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 42: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT, 2>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 82: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT, 2>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -163,9 +163,10 @@ struct SysState {
   int replacements;        // residual replacements (reliable updates / refinements)
   int maxiter;
   int since;               // iterations since the last replacement
-  int stall_iters;         // stall window: a replacement is forced after this many
-                           // iterations without one, and the solve stops (SF_STALL) when
-                           // the best true residual has not halved within this many
+  int refresh_iters;       // a residual replacement is forced after this many iterations
+                           // without one (keeps the best-iterate record current)
+  int stall_iters;         // stall window (0 = off): the solve stops (SF_STALL) when the
+                           // best true residual has not halved within this many iterations
   int stalls;              // consecutive replacements whose recursive residual had drifted
                            // >= 100x below the true one (precision exhausted)
   int stall_limit;         // stop (SF_STALL) after this many such replacements

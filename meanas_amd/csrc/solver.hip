@@ -97,6 +97,14 @@ OpView<T> view_of(const mx_operator& op) {
 // The automatic tile height is fixed: a system's block partition (and with it the order
 // of every reduction) must depend on that system alone, never on the rest of a batch.
 constexpr int kAutoTileRows = 4;
+constexpr int kStallWindow = 2000;    // default stall window of MX_MIXED
+constexpr int kRefreshIters = 2000;   // forced FP64 residual replacement cadence
+
+// Replacement cadence: the stall rule is evaluated at replacements, so they come at
+// least once per stall window.
+int refresh_iters(int stall_iters) {
+  return stall_iters > 0 ? std::min(stall_iters, kRefreshIters) : kRefreshIters;
+}
 
 int auto_slab(const mx_operator& op, int ty) {
   // enough blocks to fill 256 CUs several times over, but deep slabs to amortise the
@@ -218,7 +226,7 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     SysDesc d;
     std::memset(&d, 0, sizeof(d));
     d.op = y.op;
-    d.slab = o.slab > 0 ? std::min(o.slab, op.nx) : auto_slab(op, ty);
+    d.slab = y.slab > 0 ? std::min(y.slab, op.nx) : o.slab > 0 ? std::min(o.slab, op.nx) : auto_slab(op, ty);
     d.nblk_apply = apply_blocks(op, ty, d.slab);
     d.n3 = 3 * ncell_of(op);
     d.nblk_vec = vec_blocks(d.n3);
@@ -350,7 +358,10 @@ mx_options defaults(const mx_options* opt) {
   if (o.chunk <= 0) o.chunk = 16;
   if (o.check_every <= 0) o.check_every = (o.mode == MX_MIXED) ? 1 : o.chunk;
   if (o.delta <= 0) o.delta = (o.mode == MX_MIXED) ? 0.1 : 0.0;
-  if (o.stall_iters == 0) o.stall_iters = 2000;
+  // The stall window only pays where a stall hands the system to a stronger solver
+  // (mixed -> FP64).  FP64 COCG on PML problems crosses plateaus of several thousand
+  // iterations, so FP64 runs to maxiter unless the caller sets a window.
+  if (o.stall_iters == 0) o.stall_iters = (o.mode == MX_MIXED) ? kStallWindow : -1;
   if (o.stall_limit == 0) o.stall_limit = 3;
   return o;
 }
@@ -527,6 +538,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     st.maxiter = std::max(1, systems[i].maxiter);
     st.delta2 = o.delta * o.delta;
     st.stall_iters = std::max(0, o.stall_iters);
+    st.refresh_iters = refresh_iters(o.stall_iters);
     st.stall_limit = std::max(0, o.stall_limit);
     maxiter_all = std::max(maxiter_all, st.maxiter);
   }
@@ -766,7 +778,8 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
   d.op = slot;   // one operator per slot
-  d.slab = S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty);
+  d.slab = y->slab > 0 ? std::min<int>(y->slab, op->nx)
+         : S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty);
   d.nblk_apply = apply_blocks(*op, S->ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   d.nblk_vec = vec_blocks(d.n3);
@@ -793,6 +806,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   st->maxiter = std::max(1, y->maxiter);
   st->delta2 = S->o.delta * S->o.delta;
   st->stall_iters = std::max(0, S->o.stall_iters);
+  st->refresh_iters = refresh_iters(S->o.stall_iters);
   st->stall_limit = std::max(0, S->o.stall_limit);
   OpView<double> v64 = view_of<double>(*op);
   OpView<float> v32;

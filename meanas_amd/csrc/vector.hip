@@ -395,7 +395,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
       f |= SF_NAN | SF_PAUSE | SF_LIMIT;
     } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
-               (st->stall_iters > 0 && st->since >= st->stall_iters)) {
+               (st->refresh_iters > 0 && st->since >= st->refresh_iters)) {
       f |= SF_PAUSE;
     }
     if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;

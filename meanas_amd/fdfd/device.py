@@ -257,7 +257,8 @@ def solve_batch(ops: Sequence[DeviceOperator], systems: Sequence[dict], mode: in
 
 
 def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
-    o = dict(tile_rows=0, slab=0, chunk=0, check_every=0, use_graph=1, delta=0.0, stall_iters=0, stall_limit=0)
+    o = dict(tile_rows=0, slab=0, chunk=0, check_every=0, use_graph=1, delta=0.0, stall_iters=0, stall_limit=0,
+             autotune=True)
     if options:
         o.update(options)
     descs = (_lib.MxOperator * len(ops))(*[op.desc for op in ops])
@@ -271,7 +272,11 @@ def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
         if work is None:
             work = torch.empty(op.work_bytes(mode), dtype=torch.uint8, device=op.device)
         keep.append(work)
-        sysarr[i] = _lib.MxSystem(s['op'], int(s.get('maxiter', 10 * op.n3)), float(s.get('rtol', 1e-5)),
+        slab = int(s.get('slab', 0))
+        if slab == 0 and o['slab'] == 0 and o.get('autotune', True):
+            from .autotune import tuned_slab
+            slab = tuned_slab(op, mode)
+        sysarr[i] = _lib.MxSystem(s['op'], int(s.get('maxiter', 10 * op.n3)), slab, 0, float(s.get('rtol', 1e-5)),
                                   float(s.get('atol', 0.0)), s['b'].data_ptr(), s['x'].data_ptr(), work.data_ptr())
     opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'], o['delta'],
                           o['stall_iters'], o['stall_limit'])

@@ -131,8 +131,9 @@ class Server:
     def admit(self, slot: _Slot, now: float) -> int:
         req = slot.req
         maxiter = req.maxiter if req.maxiter is not None else 10 * slot.op.n3
-        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), req.rtol, req.atol, slot.b.data_ptr(),
-                             slot.x.data_ptr(), slot.work.data_ptr())
+        from .fdfd.autotune import tuned_slab
+        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), tuned_slab(slot.op, self.mode), 0, req.rtol, req.atol,
+                             slot.b.data_ptr(), slot.x.data_ptr(), slot.work.data_ptr())
         idx = _lib.lib().mx_session_admit(self.handle, ctypes.byref(slot.op.desc), ctypes.byref(sysd),
                                           torch_stream_handle())
         if idx < 0:
