@@ -79,11 +79,14 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   SysState* st = S.st;
-  bool mr = false;   // minimal-residual step: accumulate q^H p and ||q||^2 instead of p^T q
+  bool mr = false;     // rescue step: accumulate ||q||^2 instead of p^T q, q^T q
+  bool keep_p = false;  // residual replacement that keeps the search direction
   if (EPI != EPI_PLAIN) {
     const int f = gload1<int>(&st->flags, 0);
     if ((f & skip_flags) || (need_flags && !(f & need_flags))) return;
     mr = (f & SF_MR) != 0;
+    // fused iteration form only: the classic form rebuilds p in k_pupdate
+    keep_p = EPI == EPI_RESID && (f & SF_FUSED) && !(f & (SF_MR | SF_RESTART)) && S.vec[VS_P] != nullptr;
   }
   const OpView<T> op = ops[S.op];
   const int nx = op.nx, ny = op.ny, nz = op.nz;
@@ -201,13 +204,14 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
     __syncthreads();
   }
 
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const cplx<T> w2 = op.w2;
   const bool diag_real = !(op.flags & OPF_EPS_COMPLEX) && w2.im == T(0);
   int cb = 1;
   T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
+  RT* __restrict__ pdir = reinterpret_cast<RT*>(S.vec[VS_P]);
 
   auto step = [&](int i, Regs& R) {
     const int nb = cb ^ 1;
@@ -309,6 +313,12 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
           const cplx<double> bb = gload<double>(bvec, idx);
           const cplx<double> rr = bb - cast<double>(out[c]);
           const cplx<RT> rs = cast<RT>(rr);
+          if (keep_p) {
+            // p was built from the recursive residual: p += r_true - r_recursive keeps
+            // its direction component and swaps in the true residual
+            const cplx<RT> rold = gload<RT>(rout, idx);
+            gstore<RT>(pdir, idx, gload<RT>(pdir, idx) + (rs - rold));
+          }
           gstore<RT>(rout, idx, rs);
           const cplx<double> rd = cast<double>(rs);
           acc[0] += norm2(rr);
@@ -319,13 +329,13 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
           if constexpr (EPI == EPI_COCG) {
             if (!mr) {
               const cplx<T> pq = ec[c] * out[c];
+              const cplx<T> qq = out[c] * out[c];
               acc[0] += (double)pq.re;
               acc[1] += (double)pq.im;
+              acc[2] += (double)qq.re;
+              acc[3] += (double)qq.im;
             } else {
-              const cplx<double> qd = cast<double>(out[c]), pd = cast<double>(ec[c]);
-              acc[0] += qd.re * pd.re + qd.im * pd.im;
-              acc[1] += qd.re * pd.im - qd.im * pd.re;
-              acc[2] += norm2(qd);
+              acc[2] += norm2(cast<double>(out[c]));
             }
           }
         }
@@ -342,9 +352,13 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
     }
   }
 
-  if constexpr (EPI != EPI_PLAIN) {
-    block_sum<3>(acc, s_red);
-    store_partials<3>(acc, S.partial, bm.local, S.nblk_apply);
+  if constexpr (EPI == EPI_COCG) {
+    block_sum<4>(acc, s_red);
+    store_partials<4>(acc, S.partial, bm.local, S.nblk_apply);
+  } else if constexpr (EPI == EPI_RESID) {
+    double a3[3] = {acc[0], acc[1], acc[2]};
+    block_sum<3>(a3, s_red);
+    store_partials<3>(a3, S.partial, bm.local, S.nblk_apply);
   }
 }
 

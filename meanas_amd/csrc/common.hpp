@@ -136,6 +136,7 @@ enum : int {
   SF_MR        = 256,  // next step is a rescue step along a hashed direction (breakdown)
   SF_STALL     = 512,  // stopped: replacements no longer reduce the true residual
   SF_SAVE      = 1024, // the last replacement found the best true residual: save x
+  SF_FUSED     = 2048, // fused iteration form (complex128 inner vectors; see vector.hip)
 };
 
 // |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown
@@ -145,9 +146,9 @@ constexpr double kRhoBreakdown = 1e-10;
 // follows each reducing kernel (deterministic fixed-order sums), read by later kernels.
 struct SysState {
   cplx<double> rho;        // r^T r (unconjugated) of the current residual
-  cplx<double> rho_old;    // previous rho (for beta after a residual replacement)
   cplx<double> alpha, beta;
   cplx<double> mu;         // p^T A p
+  cplx<double> rho_old;    // previous rho (classic form: beta after a residual replacement)
   double rnorm2;           // ||r||^2 of the (recursive) residual
   double rmax2;            // max recursive ||r||^2 since the last residual replacement
   double true_rnorm2;      // ||b - A x||^2 in FP64 at the last replacement

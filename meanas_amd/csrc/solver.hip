@@ -23,8 +23,8 @@ hipError_t launch_apply_cocg_f64(int ty, const OpView<double>*, const SysDesc*, 
 hipError_t launch_apply_resid_r32(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_apply_resid_r64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 // vector.hip
-hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
+hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_update_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_update_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_accum_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
@@ -289,16 +289,16 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
   return 0;
 }
 
-// one COCG iteration over every runnable system of the batch
+// one COCG iteration over every runnable system of the batch (the direction update
+// launch only acts on systems in the classic form; see vector.hip)
 int enqueue_iteration(Batch& B, hipStream_t s) {
-  const bool mixed = B.mode == MX_MIXED;
-  if (mixed) {
-    MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  if (B.mode == MX_MIXED) {
+    MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
-    MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+    MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
@@ -307,7 +307,8 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
   return 0;
 }
 
-// residual replacement on paused systems: x += x_lo ; r = b - A x (FP64)
+// residual replacement on paused systems: x += x_lo ; r = b - A x (FP64); then the
+// search direction of fused-form systems that restart or take a rescue step
 int enqueue_replacement(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_accum_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
@@ -318,6 +319,7 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
   }
   MX_TRY(launch_fin(FIN_RESID, B.d_sys, B.d_syslist, B.nsl, s));
   MX_TRY(launch_save_best(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  if (B.mode == MX_FP64) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 1, s));
   return 0;
 }
 
@@ -492,7 +494,7 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   d.partial = reinterpret_cast<double*>(mem + s_op + s_sys + s_st);
   SysState st;
   std::memset(&st, 0, sizeof(st));
-  st.flags = SF_PAUSE;
+  st.flags = SF_PAUSE | SF_RESTART;   // RESTART: no search direction to carry over
   st.stop2 = -1.0;
   OpView<double> v = view_of<double>(*op);
   int rc = 0;
@@ -532,7 +534,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   for (int i = 0; i < nsys; ++i) {
     SysState& st = init[i];
     std::memset(&st, 0, sizeof(st));
-    st.flags = SF_PAUSE | SF_RESTART;
+    st.flags = SF_PAUSE | SF_RESTART | (o.mode == MX_FP64 ? SF_FUSED : 0);
     st.rtol = systems[i].rtol;
     st.atol = systems[i].atol;
     st.maxiter = std::max(1, systems[i].maxiter);
@@ -577,7 +579,10 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     return true;
   };
   // pipelined: keep one chunk queued behind the one whose state is being inspected
-  const long long max_chunks = (long long)maxiter_all / chunk + 4;
+  // Safety bound only: every chunk advances each unfinished system by at least one
+  // iteration or finishes it (a system that pauses for a replacement idles for the rest
+  // of that chunk, so a chunk can be worth a single iteration).
+  const long long max_chunks = (long long)maxiter_all + 4;
   if (poll(0)) return -1;
   if (launch_chunk() || poll(1)) return -1;
   int cur = 0;
@@ -800,7 +805,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   MX_TRY(hipStreamSynchronize(s));
   SysState* st = &S->h_state[slot];
   std::memset(st, 0, sizeof(*st));
-  st->flags = SF_PAUSE | SF_RESTART;
+  st->flags = SF_PAUSE | SF_RESTART | (S->o.mode == MX_FP64 ? SF_FUSED : 0);
   st->rtol = y->rtol;
   st->atol = y->atol;
   st->maxiter = std::max(1, y->maxiter);

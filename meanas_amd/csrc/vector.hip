@@ -3,18 +3,43 @@
 // (fdfd/solvers.py:115,124-125,136-139).
 //
 // COCG (van der Vorst & Melissen 1990) for A = A^T uses the unconjugated products
-// rho = r^T r and mu = p^T A p.  One iteration here is three launches:
+// rho = r^T r and mu = p^T A p.  Two forms of the iteration are implemented; each
+// system carries its form in SF_FUSED.
+//
+// Fused form (complex128 inner vectors start in it): two streaming launches, each
+// followed by a one-block-per-system finishing kernel:
+//   k_apply   : q = A p; mu = p^T q and q^T q                        (apply.hip)
+//   k_fin<0>  : alpha = rho / mu; beta = (alpha^2 q^T q - rho) / rho
+//   k_update  : x_lo += alpha p, r -= alpha q, p = r + beta p; rho' = r^T r, ||r||^2
+//   k_fin<1>  : rho = rho', convergence and replacement checks
+// beta is the value that makes the new p A-conjugate to the old one: with r^T A p =
+// p^T A p (conjugacy of the previous direction), r'^T r' = rho - 2 alpha mu + alpha^2
+// q^T q = alpha^2 q^T q - rho.  Knowing beta before r' is reduced lets the direction
+// update ride in the same pass as x and r: 9 vector streams per iteration instead of
+// 11 (Chronopoulos & Gear 1989 use the same identity).  The exact rho' still comes
+// from the stored r' and is what alpha uses.
+//
+// Classic form (complex64 inner vectors; complex128 systems whose recursion lost track):
+// the prediction above cancels (|r^T r| is often 1e-3 ||r||^2 for these complex-symmetric
+// operators), which complex64 cannot afford, and its accuracy floor is worse on nearly
+// singular operators (near-static omega).  beta = rho' / rho from the stored r' and p
+// is updated in its own pass:
 //   k_pupdate : p = r + beta p              (p = r after a restart)
-//   k_apply   : q = A p, mu = p^T q, alpha = rho / mu      (apply.hip)
-//   k_update  : x_lo += alpha p, r -= alpha q, rho' = r^T r, ||r||^2, beta = rho'/rho
+//   k_apply, k_fin<0> (alpha), k_update (x_lo, r, rho', ||r||^2), k_fin<1> (beta)
+//
 // Residual replacement (Sleijpen & van der Vorst 1996 "reliable updates"; the mixed
 // precision form of Clark et al. 2010): when the recursive ||r|| has dropped by
-// `delta` since the last replacement, or reached the target, the system pauses and
+// `delta` since the last replacement, reached the target, or `refresh_iters` passed,
+// the system pauses and
 //   k_accum   : x += x_lo (FP64), x_lo = 0
-//   k_apply<RESID> : r = b - A x in FP64, ||r||^2 in FP64, rho' from the stored r
-// replace the recursive residual with the true one and keep the search direction
-// (beta = rho'/rho_old).  Every scalar is reduced in a fixed order by the last block
-// to arrive, so results do not depend on block scheduling or on batch composition.
+//   k_apply<RESID> : r = b - A x in FP64, ||r||^2 in FP64, rho from the stored r
+// replace the recursive residual with the true one and keep the search direction:
+// the fused form adds r_true - r_recursive to p in the same pass (k_pupdate then only
+// runs for a restart or a rescue step); the classic form sets beta = rho'/rho_old for
+// the next k_pupdate.  A fused system whose true residual at a replacement exceeds 10x
+// the recursive one switches to the classic form (with a restart).  Every scalar is reduced in a fixed order (per-block partials,
+// then one finishing block per system), so results do not depend on block scheduling
+// or on the batch.
 #include "common.hpp"
 
 namespace mx {
@@ -71,14 +96,19 @@ struct Trip {
   __device__ __forceinline__ long long at(int u) const { return base + (long long)u * VB; }
 };
 
-// ---- p = r + beta p ------------------------------------------------------------
+// ---- search direction -------------------------------------------------------------
+// Fused form: only after a replacement (post = 1), for a restart (p = r) or a rescue
+// step (p = g).  Classic form: at the start of every iteration (post = 0), p = r + beta p
+// (or p = r, or p = g).
 template <typename T>
 __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ ops, const SysDesc* __restrict__ systems,
-                                              const BlockMap* __restrict__ bmap) {
+                                              const BlockMap* __restrict__ bmap, int post) {
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   const int f = gload1<int>(&S.st->flags, 0);
   if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  const bool fused = (f & SF_FUSED) != 0;
+  if (fused ? (!post || !(f & (SF_MR | SF_RESTART))) : post) return;
   const cplx<T>* __restrict__ r = reinterpret_cast<const cplx<T>*>(S.vec[VS_R]);
   cplx<T>* __restrict__ p = reinterpret_cast<cplx<T>*>(S.vec[VS_P]);
   const long long n = S.n3;
@@ -120,24 +150,23 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
     p[n - 1] = restart ? r[n - 1] : r[n - 1] + beta * p[n - 1];
 }
 
-// ---- x_lo += alpha p ; r -= alpha q ; rho, ||r||^2 ---------------------------------
-template <typename T>
-__global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
+// ---- x_lo += alpha p ; r -= alpha q ; [p = r + beta p] ; rho, ||r||^2 ----------------
+template <typename T, bool kFused>
+__device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm) {
   __shared__ double s_red[3 * 16];
-  const BlockMap bm = bmap[blockIdx.x];
-  const SysDesc& S = systems[bm.sys];
   SysState* st = S.st;
-  if (skip_sys(gload1<int>(&st->flags, 0), SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
   cplx<T>* __restrict__ x = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   cplx<T>* __restrict__ r = reinterpret_cast<cplx<T>*>(S.vec[VS_R]);
-  const cplx<T>* __restrict__ p = reinterpret_cast<const cplx<T>*>(S.vec[VS_P]);
+  cplx<T>* __restrict__ p = reinterpret_cast<cplx<T>*>(S.vec[VS_P]);
   const cplx<T>* __restrict__ q = reinterpret_cast<const cplx<T>*>(S.vec[VS_Q]);
   const cplx<T> alpha = cast<T>(gload<double>(&st->alpha, 0));
+  const cplx<T> beta = kFused ? cast<T>(gload<double>(&st->beta, 0)) : zero<T>();
   const long long n = S.n3;
   T rr_re = 0, rr_im = 0, nr = 0;
-  auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T> pv, cplx<T> qv) {
+  auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T>& pv, cplx<T> qv) {
     xv = xv + alpha * pv;
     rv = rv - alpha * qv;
+    if constexpr (kFused) pv = rv + beta * pv;
     rr_re += rv.re * rv.re - rv.im * rv.im;
     rr_im += T(2) * rv.re * rv.im;
     nr += rv.re * rv.re + rv.im * rv.im;
@@ -162,17 +191,29 @@ __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ syste
         for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
         stp<T>(x, t.at(u), xv[u]);
         stp<T>(r, t.at(u), rv[u]);
+        if constexpr (kFused) stp<T>(p, t.at(u), pv[u]);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
-    cplx<T> xe = x[n - 1], re = r[n - 1];
-    one(xe, re, p[n - 1], q[n - 1]);
+    cplx<T> xe = x[n - 1], re = r[n - 1], pe1 = p[n - 1];
+    one(xe, re, pe1, q[n - 1]);
     x[n - 1] = xe;
     r[n - 1] = re;
+    if constexpr (kFused) p[n - 1] = pe1;
   }
   double acc[3] = {(double)rr_re, (double)rr_im, (double)nr};
   block_sum<3>(acc, s_red);
   store_partials<3>(acc, S.partial, bm.local, S.nblk_vec);
+}
+
+template <typename T>
+__global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
+  const BlockMap bm = bmap[blockIdx.x];
+  const SysDesc& S = systems[bm.sys];
+  const int f = gload1<int>(&S.st->flags, 0);
+  if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  if (sizeof(T) == 8 && (f & SF_FUSED)) update_body<T, true>(S, bm);   // the direction update rides along
+  else update_body<T, false>(S, bm);
 }
 
 // ---- x (c128) += x_lo ; x_lo = 0   (paused systems) --------------------------------
@@ -326,7 +367,7 @@ enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3 };
 
 template <int KIND>
 __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
-  __shared__ double s_red[3 * 16];
+  __shared__ double s_red[4 * 16];
   const SysDesc& S = systems[syslist[blockIdx.x]];
   SysState* st = S.st;
   int f = st->flags;
@@ -337,36 +378,53 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
   } else {
     if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
   }
+  const bool fused = (f & SF_FUSED) != 0;
   const int nblk = (KIND == FIN_APPLY || KIND == FIN_RESID) ? S.nblk_apply : S.nblk_vec;
-  double tot[3] = {0.0, 0.0, 0.0};
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (KIND == FIN_BNORM) {
     double t1[1];
     final_sum<1>(t1, S.partial, nblk, s_red);
     tot[0] = t1[0];
+  } else if constexpr (KIND == FIN_APPLY) {
+    final_sum<4>(tot, S.partial, nblk, s_red);
   } else {
-    final_sum<3>(tot, S.partial, nblk, s_red);
+    double t3[3];
+    final_sum<3>(t3, S.partial, nblk, s_red);
+    tot[0] = t3[0]; tot[1] = t3[1]; tot[2] = t3[2];
   }
   if (threadIdx.x != 0) return;
 
   if constexpr (KIND == FIN_APPLY) {
     if (f & SF_MR) {
       // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
-      // new residual r - alpha A g generic (r^T r no longer ~ 0).  SF_RESTART stays set:
-      // COCG restarts from the new residual.
+      // new residual r - alpha A g generic (r^T r no longer ~ 0); beta = 0 restarts
+      // COCG from that residual.
       const double qq = tot[2];
       const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
       if (!(qq > 0.0) || !finite(alpha.re)) f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0
-      else st->alpha = alpha;
+      else {
+        st->alpha = alpha;
+        if (fused) {   // beta = 0 restarts the fused recurrence (classic: k_pupdate does)
+          st->beta = cplx<double>{0.0, 0.0};
+          f &= ~SF_RESTART;
+        }
+      }
     } else {
-      const cplx<double> mu{tot[0], tot[1]};
+      const cplx<double> mu{tot[0], tot[1]}, qtq{tot[2], tot[3]};
       st->mu = mu;
-      const cplx<double> alpha = cdiv(st->rho, mu);
+      const cplx<double> rho = st->rho;
+      const cplx<double> alpha = cdiv(rho, mu);
       if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
         // p^T A p vanished (or overflowed): replace the residual, then rescue with a
         // step along a hashed direction and restart from p = r
         f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
         if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
       } else {
+        if (fused) {   // predicted rho' = alpha^2 q^T q - rho; beta = rho' / rho
+          cplx<double> beta = cdiv(alpha * alpha * qtq, rho) - cplx<double>{1.0, 0.0};
+          if (!finite(beta.re) || !finite(beta.im)) beta = cplx<double>{0.0, 0.0};
+          st->beta = beta;
+        }
         st->alpha = alpha;
         f &= ~SF_RESTART;
       }
@@ -375,17 +433,21 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     const cplx<double> rho{tot[0], tot[1]};
     const double rn2 = tot[2];
     f &= ~SF_SAVE;     // any saved best iterate was stored by k_save_best before this
-    if (f & SF_MR) {   // that was a rescue step: COCG restarts from p = r
-      f &= ~SF_MR;
-      st->beta = cplx<double>{0.0, 0.0};
-    } else if (norm2(st->rho) > 0.0) {
-      st->beta = cdiv(rho, st->rho);
-    } else {           // rho breakdown: restart from the replaced residual
-      f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
-      st->beta = cplx<double>{0.0, 0.0};
+    const bool rescued = (f & SF_MR) != 0;
+    f &= ~SF_MR;       // a rescue step (if that was one) is done
+    if (!fused) {      // classic: beta from the exact rho'; a rescue step restarts
+      if (rescued) st->beta = cplx<double>{0.0, 0.0};
+      else if (norm2(st->rho) > 0.0) st->beta = cdiv(rho, st->rho);
+      else {           // rho breakdown: restart from the replaced residual
+        f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
+        st->beta = cplx<double>{0.0, 0.0};
+      }
     }
-    // r^T r ~ 0 with r != 0: the next step is a rescue step from a hashed direction
-    if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0) f |= SF_MR | SF_RESTART;
+    // r^T r ~ 0 with r != 0: the next step is a rescue step along a hashed direction.
+    // The fused form has already built p from r, so it pauses for a replacement, after
+    // which k_pupdate writes the rescue direction.
+    if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0)
+      f |= SF_MR | SF_RESTART | (fused ? SF_PAUSE : 0);
     st->rho_old = st->rho;
     st->rho = rho;
     st->rnorm2 = rn2;
@@ -424,14 +486,22 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
       st->best_true2 = rn2;
       f |= SF_SAVE;
     }
+    // a fused recursion that lost track of the true residual continues in the classic
+    // form, from a restart (p was built with the predicted beta)
+    if (fused && !first && st->rnorm_rec2 < 1e-2 * rn2) f = (f & ~SF_FUSED) | SF_RESTART;
     if (st->stall_limit > 0 && st->stalls >= st->stall_limit) f |= SF_STALL | SF_DONE;
     if (st->stall_iters > 0 && st->iters - st->mark_iter > st->stall_iters) f |= SF_STALL | SF_DONE;
     if (!finite(rn2)) f |= SF_NAN | SF_DONE;
     if (rn2 <= st->stop2 || (f & (SF_DONE | SF_LIMIT))) {
       f |= SF_DONE;
     } else {
-      const cplx<double> ro = st->rho_old;
-      st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
+      // fused: the replacement kernel already moved p onto the true residual (unless a
+      // restart or a rescue step is pending: k_pupdate rebuilds p then).  Classic: the
+      // next k_pupdate continues the direction with beta = rho_true / rho_old.
+      if (!(f & SF_FUSED)) {
+        const cplx<double> ro = st->rho_old;
+        st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
+      }
       st->rho = rho;
       if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2) f |= SF_RESTART | SF_MR;
       f &= ~(SF_PAUSE | SF_BREAKDOWN);
@@ -459,14 +529,18 @@ MX_VEC_LAUNCH(launch_bnorm, k_bnorm)
 MX_VEC_LAUNCH(launch_save_best, k_save_best)
 #undef MX_VEC_LAUNCH
 
-hipError_t launch_pupdate_f32(const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+// post = 0: the iteration's direction update (classic form); post = 1: after a
+// residual replacement (restart or rescue step of the fused form)
+hipError_t launch_pupdate_f32(const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, int post,
+                              hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pupdate<float>, dim3(nb), dim3(VB), 0, s, ops, sys, bmap);
+  hipLaunchKernelGGL(k_pupdate<float>, dim3(nb), dim3(VB), 0, s, ops, sys, bmap, post);
   return hipGetLastError();
 }
-hipError_t launch_pupdate_f64(const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+hipError_t launch_pupdate_f64(const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, int post,
+                              hipStream_t s) {
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pupdate<double>, dim3(nb), dim3(VB), 0, s, ops, sys, bmap);
+  hipLaunchKernelGGL(k_pupdate<double>, dim3(nb), dim3(VB), 0, s, ops, sys, bmap, post);
   return hipGetLastError();
 }
 
