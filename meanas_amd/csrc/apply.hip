@@ -8,11 +8,12 @@
 //
 // Kernel shape: a block owns a TY x 64 tile of (y, z) columns and marches along x over
 // `slab` planes.  Per plane it reads the 3 components of the next x-plane of the unknown
-// (tile + one-cell halo in y and z) once from HBM into a double-buffered LDS plane,
-// computes H on the tile plus a one-cell halo on the minus side into LDS, and forms the
-// output from H(i), its y-/z- neighbours in LDS and H(i-1) kept in registers.  z is the
-// contiguous axis: a wave reads 64 consecutive complex values per row.
-#include <cstdlib>
+// (tile + one-cell halo in y and z) once from HBM into registers; the H phase of plane i
+// takes its y/z neighbours from plane i in LDS and the x-difference from plane i+1 in
+// registers, then plane i+1 replaces plane i in the single LDS plane.  H on the tile plus
+// a one-cell halo on the minus side goes to LDS, and the output is formed from H(i), its
+// y-/z- neighbours in LDS and H(i-1) kept in registers.  z is the contiguous axis: a wave
+// reads 64 consecutive complex values per row.
 #include <type_traits>
 
 #include "common.hpp"
@@ -49,30 +50,35 @@ __device__ __forceinline__ cplx<T> dif(const cplx<T>& fa, const cplx<T>& x1, con
 }
 template <typename T> __device__ __forceinline__ bool is_real(const cplx<T>& a) { return a.im == T(0); }
 
-// forward-difference coefficients (fa, fb) of entry n of a per-axis table
+// forward-difference coefficients (fa, fb) of entry n of a per-axis table; coef_u for
+// an n that is uniform over the wave (scalar loads, SGPRs)
 template <typename T> struct FaFb { cplx<T> fa, fb; };
 template <typename T>
 __device__ __forceinline__ FaFb<T> coef_at(const AxisCoef<T>* tab, int n) {
   return FaFb<T>{gload<T>(tab, 4 * n), gload<T>(tab, 4 * n + 1)};
 }
+template <typename T>
+__device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
+  return FaFb<T>{cload<T>(tab, 4 * n), cload<T>(tab, 4 * n + 1)};
+}
 
-// Occupancy target: 4 waves per SIMD for complex64 (<= 128 VGPRs); complex128 needs more
-// than 168 VGPRs, so forcing 3 waves would spill.
+// Occupancy target (the second launch-bounds argument is the minimum number of waves
+// per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs about 190 VGPRs,
+// and holding it to 3 waves (168) spills, so it runs 2.
 template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : 2;
 
-// Prefetch depth (planes of loads in flight beyond the one being committed).
-template <typename T, int TY, int EPI, typename RT, int PD>
-__global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) / TY : 1) void k_apply(const OpView<T>* __restrict__ ops,
-                                                   const SysDesc* __restrict__ systems,
-                                                   const BlockMap* __restrict__ bmap,
-                                                   int in_slot, int out_slot,
-                                                   int skip_flags, int need_flags,
-                                                   const void* in_ptr, void* out_ptr) {
+template <typename T, int TY, int EPI, typename RT>
+__global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpView<T>* __restrict__ ops,
+                                                                 const SysDesc* __restrict__ systems,
+                                                                 const BlockMap* __restrict__ bmap,
+                                                                 int in_slot, int out_slot,
+                                                                 int skip_flags, int need_flags,
+                                                                 const void* in_ptr, void* out_ptr) {
   constexpr int TZ = 64;
   constexpr int LY = TY + 2, LZ = TZ + 2;
-  static_assert(TY >= 2, "tile needs at least two rows");
-  __shared__ cplx<T> sE[2][3][LY][LZ];
-  __shared__ uint8_t sM[2][LY][LZ];
+  static_assert(TY >= 2 && 2 * TY + 2 <= 64, "tile rows: the z-halo loads take 2 TY + 2 lanes of one wave");
+  __shared__ cplx<T> sE[3][LY][LZ];   // plane i of the unknown, tile + one-cell halo
+  __shared__ uint8_t sM[LY][LZ];      // its PEC/PMC mask bits
   __shared__ cplx<T> sH[3][TY + 1][TZ + 1];
   __shared__ double s_red[4 * 16];
 
@@ -98,7 +104,8 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
   const int j0 = (tile / ntk) * TY, k0 = (tile % ntk) * TZ;
   const int ib = slab_id * S.slab;
   const int ie = min(ib + S.slab, nx);
-  const int tz = threadIdx.x, ty = threadIdx.y;
+  // threadIdx.y is uniform over a wave (blockDim.x = 64); readfirstlane tells the compiler
+  const int tz = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int j = j0 + ty, k = k0 + tz;
   const bool valid = (j < ny) && (k < nz);
   const int jw = wrapi(j, ny), kw = wrapi(k, nz);
@@ -112,20 +119,27 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
   const T* __restrict__ xin = reinterpret_cast<const T*>(in_ptr ? in_ptr : S.vec[in_slot]);
 
   // ---- load sites of this thread: own column, optional y-halo row, optional z-halo
-  // column.  Indices are 32-bit: the host guarantees 3N < 2^31.
+  // cell.  Indices are 32-bit: the host guarantees 3N < 2^31.
+  // Only plane i sits in LDS; the x-difference takes plane i+1 from the registers of the
+  // thread that loaded it, so every value of plane i+1 that a thread needs is one of its
+  // own loads: its own cell, its y-halo row (ty == 0 computes the halo H of row j0-1) and
+  // its z-halo cell (lanes 0..TY-1 of wave TY-1 compute the halo H of column k0-1).
   const int own_off = jw * nz + kw;
   const int own_lds = (ty + 1) * LZ + (tz + 1);
   const bool has_row = (ty == 0) || (ty == TY - 1);
   const int row_off = wrapi(ty == 0 ? j0 - 1 : j0 + TY, ny) * nz + kw;
   const int row_lds = (ty == 0 ? 0 : TY + 1) * LZ + (tz + 1);
-  // z-halo columns (k0-1 and k0+TZ) for rows 0..TY+1: lanes of wave TY/2
-  const bool has_col = (ty == TY / 2) && (tz < 2 * LY);
-  const int col_r = tz >> 1, col_side = tz & 1;
-  const int col_off = has_col ? wrapi(j0 - 1 + col_r, ny) * nz + wrapi(col_side ? k0 + TZ : k0 - 1, nz) : 0;
-  const int col_lds = col_r * LZ + (col_side ? TZ + 1 : 0);
+  // z-halo cells, wave TY-1: lanes 0..TY take column k0-1 at LDS rows 1..TY+1, lanes
+  // TY+1..2TY+1 column k0+TZ at LDS rows 0..TY (the halo cells the stencil reads)
+  const bool has_col = (ty == TY - 1) && (tz < 2 * TY + 2);
+  const int col_r = tz < TY + 1 ? tz + 1 : tz - (TY + 1);
+  const int col_c = tz < TY + 1 ? 0 : TZ + 1;
+  const int col_off = has_col ? wrapi(j0 - 1 + col_r, ny) * nz + wrapi(col_c ? k0 + TZ : k0 - 1, nz) : 0;
+  const int col_lds = col_r * LZ + col_c;
+  const bool col_halo_h = (ty == TY - 1) && (tz < TY);   // halo H at column k0-1, LDS row tz+1
   const int nc = (int)ncell;
 
-  // One plane's worth of this thread's loads (own column, y-halo row, z-halo column).
+  // One plane's worth of this thread's loads (own column, y-halo row, z-halo cell).
   struct Regs {
     cplx<T> o[3], r[3], c[3];
     uint8_t mo, mr, mc;
@@ -146,26 +160,28 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
       R.mc = has_mask ? gload1<uint8_t>(op.mask, pb + col_off) : 0;
     }
   };
-  auto put = [&](int buf, int lds, const cplx<T> (&v)[3], uint8_t m) {
-    cplx<T>* base = &sE[buf][0][0][0];
+  // a loaded value with PEC applied (the unknown is zero on PEC components)
+  auto pec = [](const cplx<T> (&v)[3], uint8_t m, int c) { return (m >> c) & 1 ? zero<T>() : v[c]; };
+  auto put = [&](int lds, const cplx<T> (&v)[3], uint8_t m) {
+    cplx<T>* base = &sE[0][0][0];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) base[c * LY * LZ + lds] = (m >> c) & 1 ? zero<T>() : v[c];
-    (&sM[buf][0][0])[lds] = m;
+    for (int c = 0; c < 3; ++c) base[c * LY * LZ + lds] = pec(v, m, c);
+    (&sM[0][0])[lds] = m;
   };
-  auto commit = [&](const Regs& R, int buf) {
-    put(buf, own_lds, R.o, R.mo);
-    if (has_row) put(buf, row_lds, R.r, R.mr);
-    if (has_col) put(buf, col_lds, R.c, R.mc);
+  auto commit = [&](const Regs& R) {
+    put(own_lds, R.o, R.mo);
+    if (has_row) put(row_lds, R.r, R.mr);
+    if (has_col) put(col_lds, R.c, R.mc);
   };
-  // Register prefetch: the loads of planes i+2 .. i+1+PD are in flight while plane i is
-  // computed (PD register sets, used round-robin).
-  Regs RA, RB;
+  // Register prefetch: the loads of plane i+1 are in flight from the middle of step i-1
+  // until the H phase of step i.
+  Regs R;
 
   // per-thread y/z coefficients (wrapped indices)
-  const FaFb<T> cyj = coef_at(cy, jw), czk = coef_at(cz, kw);
-  const cplx<T> fa_ym = coef_at(cy, wrapi(j - 1, ny)).fa, fa_zm = coef_at(cz, wrapi(k - 1, nz)).fa;
-  const FaFb<T> cyh = coef_at(cy, wrapi(j0 - 1, ny));   // halo row j0-1
-  const FaFb<T> czh = coef_at(cz, wrapi(k0 - 1, nz));   // halo column k0-1
+  const FaFb<T> cyj = coef_u(cy, jw), czk = coef_at(cz, kw);
+  const cplx<T> fa_ym = coef_u(cy, wrapi(j - 1, ny)).fa, fa_zm = coef_at(cz, wrapi(k - 1, nz)).fa;
+  const FaFb<T> cyh = coef_u(cy, wrapi(j0 - 1, ny));   // halo row j0-1
+  const FaFb<T> czh = coef_u(cz, wrapi(k0 - 1, nz));   // halo column k0-1
 
   // H multiplier at (cell, component): pmc keep * 1/mu
   auto hscale = [&](cplx<T> h, int c, uint8_t m, int cell) -> cplx<T> {
@@ -176,63 +192,52 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
     }
     return h;
   };
+  auto e_ = [&](int c, int rr, int qq) { return sE[c][rr][qq]; };
 
-  // ---- prologue: H(ib-1) at the own column (y, z components)
+  // ---- prologue: H(ib-1) at the own column (y, z components); plane ib into LDS
   cplx<T> hpy, hpz;
   {
     const int im1 = wrapi(ib - 1, nx);
-    fetch(RA, im1);
-    commit(RA, 0);
-    fetch(RA, ib);
-    commit(RA, 1);
+    fetch(R, im1);
+    commit(R);
+    fetch(R, ib);
     __syncthreads();
-    const FaFb<T> cxm = coef_at(cx, im1);
+    const FaFb<T> cxm = coef_u(cx, im1);
     const int r = ty + 1, q = tz + 1;
-    const cplx<T>* A = &sE[0][0][0][0];
-    const cplx<T>* B = &sE[1][0][0][0];
-    auto a_ = [&](int c, int rr, int qq) { return A[(c * LY + rr) * LZ + qq]; };
-    auto b_ = [&](int c, int rr, int qq) { return B[(c * LY + rr) * LZ + qq]; };
-    cplx<T> hy = (czk.fa * a_(0, r, q + 1) - czk.fb * a_(0, r, q)) - (cxm.fa * b_(2, r, q) - cxm.fb * a_(2, r, q));
-    cplx<T> hz = (cxm.fa * b_(1, r, q) - cxm.fb * a_(1, r, q)) - (cyj.fa * a_(0, r + 1, q) - cyj.fb * a_(0, r, q));
-    const uint8_t m = sM[0][r][q];
+    const cplx<T> n1 = pec(R.o, R.mo, 1), n2 = pec(R.o, R.mo, 2);
+    cplx<T> hy = (czk.fa * e_(0, r, q + 1) - czk.fb * e_(0, r, q)) - (cxm.fa * n2 - cxm.fb * e_(2, r, q));
+    cplx<T> hz = (cxm.fa * n1 - cxm.fb * e_(1, r, q)) - (cyj.fa * e_(0, r + 1, q) - cyj.fb * e_(0, r, q));
+    const uint8_t m = sM[r][q];
     const int cell = im1 * (int)plane + own_off;
     hpy = hscale(hy, 1, m, cell);
     hpz = hscale(hz, 2, m, cell);
-    fetch(RA, wrapi(ib + 1, nx));                 // plane ib+1, committed by the first step
-    if constexpr (PD == 2)
-      if (ib + 1 < ie) fetch(RB, wrapi(ib + 2, nx));   // plane ib+2, committed by the second
     __syncthreads();
+    commit(R);                      // plane ib
+    fetch(R, wrapi(ib + 1, nx));    // plane ib+1 (H(ie-1) needs plane ie, and ib < ie)
   }
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const cplx<T> w2 = op.w2;
   const bool diag_real = !(op.flags & OPF_EPS_COMPLEX) && w2.im == T(0);
-  int cb = 1;
   T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
   RT* __restrict__ pdir = reinterpret_cast<RT*>(S.vec[VS_P]);
 
-  auto step = [&](int i, Regs& R) {
-    const int nb = cb ^ 1;
-    commit(R, nb);                                // plane i+1
-    if (i + PD < ie) fetch(R, wrapi(i + 1 + PD, nx));   // prefetch plane i+1+PD into the freed set
+#pragma unroll 1
+  for (int i = ib; i < ie; ++i) {
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       epsv[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
-    const FaFb<T> cxi = coef_at(cx, i);
-    const cplx<T> fa_xm = coef_at(cx, wrapi(i - 1, nx)).fa;
-    __syncthreads();
+    const FaFb<T> cxi = coef_u(cx, i);
+    const cplx<T> fa_xm = coef_u(cx, wrapi(i - 1, nx)).fa;
+    __syncthreads();   // plane i is in LDS; the previous step's reads of sH are done
 
-    const cplx<T>* C = &sE[cb][0][0][0];
-    const cplx<T>* N = &sE[nb][0][0][0];
-    auto e_ = [&](int c, int rr, int qq) { return C[(c * LY + rr) * LZ + qq]; };
-    auto n_ = [&](int c, int rr, int qq) { return N[(c * LY + rr) * LZ + qq]; };
     const int r = ty + 1, q = tz + 1;
     cplx<T> ec[3] = {e_(0, r, q), e_(1, r, q), e_(2, r, q)};
-    const uint8_t mown = sM[cb][r][q];
+    const uint8_t mown = sM[r][q];
     // Real-coefficient fast paths: x coefficients are uniform over the plane, y
     // coefficients over a wave (one wave is one y row); z varies per lane and stays complex.
     const bool rx = is_real(cxi.fa) && is_real(cxi.fb) && is_real(fa_xm);
@@ -241,27 +246,27 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
     auto h_phase = [&](auto RXt, auto RYt) {
       constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
       const cplx<T> hx = dif<RY>(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif<false>(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
-      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, n_(2, r, q), cxi.fb, ec[2]);
-      const cplx<T> hz = dif<RX>(cxi.fa, n_(1, r, q), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
+      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, pec(R.o, R.mo, 2), cxi.fb, ec[2]);
+      const cplx<T> hz = dif<RX>(cxi.fa, pec(R.o, R.mo, 1), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
       sH[0][r][q] = hscale(hx, 0, mown, cell);
       sH[1][r][q] = hscale(hy, 1, mown, cell);
       sH[2][r][q] = hscale(hz, 2, mown, cell);
       if (ty == 0) {   // halo row j0-1 (LDS row 0): Hx and Hz are consumed
-        const uint8_t mh = sM[cb][0][q];
+        const uint8_t mh = sM[0][q];
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
         const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
-        const cplx<T> gz = dif<RX>(cxi.fa, n_(1, 0, q), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
+        const cplx<T> gz = dif<RX>(cxi.fa, pec(R.r, R.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[0][0][q] = hscale(gx, 0, mh, hcell);
         sH[2][0][q] = hscale(gz, 2, mh, hcell);
       }
-      if (ty == TY - 1 && tz < TY) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
+      if (col_halo_h) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
         const int rr = tz + 1;
         const int jj = wrapi(j0 + tz, ny);
         const FaFb<T> cyr = coef_at(cy, jj);
-        const uint8_t mh = sM[cb][rr][0];
+        const uint8_t mh = sM[rr][0];
         const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
-        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, n_(2, rr, 0), cxi.fb, e_(2, rr, 0));
+        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(R.c, R.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
         sH[1][rr][0] = hscale(gy, 1, mh, hcell);
       }
@@ -273,7 +278,11 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
     } else {
       if (ry) h_phase(F_{}, T_{}); else h_phase(F_{}, F_{});
     }
-    __syncthreads();
+    __syncthreads();   // H(i) is in LDS; every read of plane i is done
+    if (i + 1 < ie) {
+      commit(R);                          // plane i+1 replaces plane i
+      fetch(R, wrapi(i + 2, nx));         // plane i+2
+    }
 
     const cplx<T> hx0 = sH[0][r][q], hy0 = sH[1][r][q], hz0 = sH[2][r][q];
     cplx<T> yx, yy, yz;
@@ -341,15 +350,6 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
         }
       }
     }
-    cb = nb;
-  };
-  if constexpr (PD == 1) {
-    for (int i = ib; i < ie; ++i) step(i, RA);
-  } else {
-    for (int i = ib; i < ie; i += 2) {
-      step(i, RA);
-      if (i + 1 < ie) step(i + 1, RB);
-    }
   }
 
   if constexpr (EPI == EPI_COCG) {
@@ -363,29 +363,14 @@ __global__ __launch_bounds__(64 * TY, PD == 1 ? (4 * kWavesPerSimd<T> + TY - 1) 
 }
 
 // ---- host-side launch helpers ------------------------------------------------------
-// Prefetch depth per element size; MEANAS_AMD_PREFETCH=1|2 overrides it (tuning runs).
-static int prefetch_depth(size_t elem) {
-  static int forced = [] {
-    const char* v = getenv("MEANAS_AMD_PREFETCH");
-    return v ? atoi(v) : 0;
-  }();
-  if (forced == 1 || forced == 2) return forced;
-  (void)elem;
-  return 1;   // depth 2 costs the occupancy it would buy back (measured: no gain)
-}
-
 template <typename T, int EPI, typename RT>
 static hipError_t launch_apply_ty(int ty, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
                                   const void* in_ptr = nullptr, void* out_ptr = nullptr) {
-  const int pd = prefetch_depth(sizeof(T));
-  switch (ty * 10 + pd) {
-    case 21: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT, 1>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 41: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT, 1>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 81: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT, 1>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 22: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT, 2>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 42: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT, 2>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 82: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT, 2>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+  switch (ty) {
+    case 2: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 4: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+    case 8: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

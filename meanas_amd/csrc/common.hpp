@@ -54,6 +54,15 @@ __host__ __device__ __forceinline__ cplx<double> cdiv(cplx<double> a, cplx<doubl
 #else
 #define MX_GLOBAL
 #endif
+// Read-only tables (PML coefficients) indexed by a value uniform over the wave or the
+// block go through constant-address-space pointers, so the compiler can load them with
+// scalar loads into SGPRs instead of holding a copy per lane.  (Loads only: nothing here
+// writes through the scalar path.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MX_CONST __attribute__((address_space(4)))
+#else
+#define MX_CONST
+#endif
 template <typename T> struct Vec2 { typedef T __attribute__((ext_vector_type(2))) type; };
 template <typename T> struct Vec4 { typedef T __attribute__((ext_vector_type(4))) type; };
 
@@ -74,6 +83,14 @@ __device__ __forceinline__ void gstore(void* base, int idx, cplx<T> a) {
   v.x = a.re;
   v.y = a.im;
   *(MX_GLOBAL V*)((MX_GLOBAL char*)base + off) = v;
+}
+// complex value idx of a read-only table, through the constant address space; idx must
+// be uniform over the wave for a scalar load
+template <typename T>
+__device__ __forceinline__ cplx<T> cload(const void* base, int idx) {
+  typedef typename Vec2<T>::type V;
+  const V v = ((const MX_CONST V*)base)[idx];
+  return cplx<T>{v.x, v.y};
 }
 template <typename T>
 __device__ __forceinline__ T gload1(const void* base, int idx) {
