@@ -114,6 +114,14 @@ __device__ __forceinline__ void pstore(void* base, long long i, typename PackV<T
   const unsigned off = (unsigned)i * 16u;
   *(MX_GLOBAL V*)((MX_GLOBAL char*)base + off) = v;
 }
+// streaming store: for data that is not read again before much other traffic has
+// passed (keeps it from displacing vectors the next kernel reads out of the caches)
+template <typename T>
+__device__ __forceinline__ void pstore_nt(void* base, long long i, typename PackV<T>::type v) {
+  typedef typename PackV<T>::type V;
+  const unsigned off = (unsigned)i * 16u;
+  __builtin_nontemporal_store(v, (MX_GLOBAL V*)((MX_GLOBAL char*)base + off));
+}
 
 // Per-axis coefficients at index n: fa multiplies f[n+1], fb multiplies f[n] in the
 // scaled forward difference; se = sqrt(dx_e[n]), sh = sqrt(dx_h[n]) give the diagonal

@@ -45,6 +45,10 @@
 namespace mx {
 
 constexpr int VB = 256;   // threads per vector block
+#ifndef MX_STREAM_XR
+#define MX_STREAM_XR 1
+#endif
+constexpr bool kStreamXR = MX_STREAM_XR != 0;   // streaming stores for x and r in k_update
 constexpr int VU = 4;     // 16-byte packs in flight per thread
 
 __device__ __forceinline__ bool skip_sys(int f, int skip, int need) {
@@ -69,7 +73,7 @@ __device__ __forceinline__ Pack<T> ld(const void* base, long long i) {
   }
   return p;
 }
-template <typename T>
+template <typename T, bool NT = false>
 __device__ __forceinline__ void stp(void* base, long long i, const Pack<T>& p) {
   typename PackV<T>::type w;
   if constexpr (Pack<T>::K == 2) {
@@ -77,7 +81,8 @@ __device__ __forceinline__ void stp(void* base, long long i, const Pack<T>& p) {
   } else {
     w[0] = p.v[0].re; w[1] = p.v[0].im;
   }
-  pstore<T>(base, i, w);
+  if constexpr (NT) pstore_nt<T>(base, i, w);
+  else pstore<T>(base, i, w);
 }
 
 // Contiguous pack range [*b, *e) of this block: the vector is split into nblk equal runs.
@@ -189,8 +194,9 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm)
       if (t.ok(u)) {
 #pragma unroll
         for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
-        stp<T>(x, t.at(u), xv[u]);
-        stp<T>(r, t.at(u), rv[u]);
+        // x and r are next read one iteration later; p and q are read by the next kernel
+        stp<T, kStreamXR>(x, t.at(u), xv[u]);
+        stp<T, kStreamXR>(r, t.at(u), rv[u]);
         if constexpr (kFused) stp<T>(p, t.at(u), pv[u]);
       }
   }
