@@ -5,13 +5,14 @@ Metric: solves/s to 1e-8 relative residual on the 128^3 workload (eps = 2.085 + 
 random dielectric boxes, 10-cell SC-PML on all faces, Gaussian Jy sheet at x = 20),
 plus p50/p99 solve latency and the operator-apply HBM bandwidth.
 
-A step is one complete solve (complex64 COCG with FP64 residual replacement, stopped
-when the FP64 true residual ||b - A x|| / ||b|| <= 1e-8) with the operator, materials
-and right-hand side already resident in HBM.  Beside it: the same solve with plain
-FP64 COCG (same GPU), the field error of the mixed solve against an FP64 solve to
-1e-11, the apply kernel's achieved bandwidth timed with HIP events on its own stream,
-and the oracle (restated reference: CSR assembly + scipy qmr, complex128) timed on
-this host over a bounded sample.
+A step is one complete solve (FP64 COCG by default, `--mode mixed` for complex64 COCG
+with FP64 residual replacement), stopped when the FP64 true residual ||b - A x|| / ||b||
+<= 1e-8, with the operator, materials and right-hand side already resident in HBM.
+Beside it: the same solve in the other precision mode (same GPU), the field error
+against an FP64 solve to 1e-11, each kernel of one iteration timed with HIP events on
+its launch stream (the roofline of the dominant one, and of the operator apply), and
+the oracle (restated reference: CSR assembly + scipy qmr, complex128) timed on this
+host over a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,8 +31,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # scipy qmr iterations the oracle needs on the 128^3 workload to reach rtol 1e-8,
-# measured by running the oracle to convergence on a CPU host (see DESIGN.md)
+# measured by running the oracle to convergence on a CPU host
 QMR_ITERS_128 = None
+# HBM bytes per launch (MB) from rocprofv3 FETCH_SIZE / WRITE_SIZE counters, scaled by the
+# factors a 1 GiB copy gives (profiles/*_hbm_traffic.md), keyed by kernel, mode, shape
+TRAFFIC_MB = {}
 
 
 def parse():
@@ -48,8 +52,9 @@ def parse():
     ap.add_argument('--apply-sweep', action='store_true',
                     help='only time the operator apply over tile heights and slab depths')
     ap.add_argument('--apply-only', action='store_true',
-                    help='only run the operator apply (complex64 and complex128, default tiling): '
-                         'the program the hardware-counter profiles are taken on')
+                    help='only run the operator apply (complex64 and complex128, default tiling) and the '
+                         'kernels of one FP64 solver iteration: the program the hardware-counter profiles '
+                         'are taken on')
     ap.add_argument('--workload', choices=['single', 'multi_rhs', 'uneven', 'serve'], default='single',
                     help='single: the 128^3 solve (the metric); multi_rhs: 16 right-hand sides on one '
                          '128x96x96 anisotropic structure; uneven: 32 structures of uneven size with PEC; '
@@ -133,7 +138,7 @@ def batch_workloads(args):
     """Multi-RHS, uneven multi-structure and serving workloads (one JSON line each)."""
     import torch
     from meanas_amd import _lib, workloads
-    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch, time_kernels
     dev = torch.device('cuda:0')
     mode = _lib.MX_FP64 if args.mode == 'fp64' else _lib.MX_MIXED
     if args.workload == 'multi_rhs':
@@ -215,7 +220,7 @@ def main():
 
     import torch
     from meanas_amd import _lib, workloads
-    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch, time_kernels
 
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     if args.workload != 'single':
@@ -237,6 +242,11 @@ def main():
             bpc = op.apply_bytes_per_cell(elem)
             print(json.dumps(dict(mode=mode, us=round(us, 2), bytes_per_cell=bpc,
                                   gbs=round(bpc * op.ncell / (us * 1e-6) / 1e9, 1))), flush=True)
+        # the kernels of one FP64 COCG iteration, as the solve launches them
+        J = torch.from_numpy(w['J']).to(dev)
+        b = op.map(_lib.MX_MAP_RHS, J, coef=-1j * w['omega'])
+        kt = time_kernels(op, b, _lib.MX_FP64, reps=args.apply_reps)
+        print(json.dumps(dict(iteration_kernels_us={k: round(v, 2) for k, v in kt.items()})), flush=True)
         # a plain copy of known size: reference for scaling the FETCH_SIZE / WRITE_SIZE counters
         # (1 GiB each way, far larger than the L2 and the last-level cache, so its reads miss)
         nb = 1 << 30
@@ -302,40 +312,69 @@ def main():
                       'field_error_vs_fp64_rtol_1e-11': ferr})
         del x2
 
-    # ---- apply kernel roofline: HIP events on the stream the kernel is launched on
+    # ---- kernel rooflines: each kernel of one solver iteration, timed with HIP events on
+    # the stream it is launched on (mx_time_kernels), same operator, right-hand side and
+    # tiling as the measured solves
+    stream = torch.cuda.current_stream()
+    kt = time_kernels(op, b, step_mode, reps=args.apply_reps, stream=stream)
+    fp64 = step_mode == _lib.MX_FP64
+    elem = 16 if fp64 else 8
+    n3 = op.n3
+    alg = {   # algorithmic HBM bytes per launch
+        'apply': op.apply_bytes_per_cell(elem) * op.ncell,              # p read, q written, materials
+        'update': (7 if fp64 else 6) * elem * n3,                       # x, r, p, q read; x, r (, p) written
+        'pupdate': 0 if fp64 else 3 * elem * n3,                        # fused form: no-op launch
+    }
+    names = {'apply': f'k_apply<{"double" if fp64 else "float"}, COCG epilogue>',
+             'update': f'k_update<{"double" if fp64 else "float"}>{" (fused direction update)" if fp64 else ""}',
+             'pupdate': f'k_pupdate<{"double" if fp64 else "float"}>'}
+    it_us = sum(kt[k] for k in ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate'))
+    kern = {k: dict(us=round(kt[k], 2), share=round(kt[k] / it_us, 3)) for k in
+            ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')}
+    for k, v in alg.items():
+        if v:
+            kern[k].update(bytes=v, gbs=round(v / (kt[k] * 1e-6) / 1e9, 1))
+    dom = max(('apply', 'update'), key=lambda k: kt[k])
+    log(f'iteration kernels (us): {kern}; dominant {dom}')
+
+    def roof(k):
+        ach = alg[k] / (kt[k] * 1e-6) / 1e9
+        tr = TRAFFIC_MB.get((k, 'fp64' if fp64 else 'mixed', tuple(op.shape)))
+        return {'bound': 'hbm', 'achieved': ach, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': ach / HBM_PEAK_GBS,
+                'traffic': tr * 1e6 if tr else None, 'kernel': names[k], 'us_per_launch': kt[k],
+                'algorithmic_bytes': alg[k], 'timing': 'HIP events on the launch stream (mx_time_kernels)'}
+    extra['roofline_apply'] = roof('apply')
+    extra['iteration_kernels'] = kern
+    extra['tiling'] = dict(tile_rows=4, slab=kt['slab'])
+    # complex64 plain apply (the mixed solver's inner operator) for reference
     g = torch.Generator(device=dev).manual_seed(0)
     xa = torch.randn(op.n3, dtype=torch.complex64, device=dev, generator=g)
     ya = torch.empty_like(xa)
-    plan = op.plan('c64')
-    stream = torch.cuda.current_stream()
-    for _ in range(10):
-        plan.apply(xa, ya, stream=stream)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.apply_reps):
-        plan.apply(xa, ya, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    apply_us = e0.elapsed_time(e1) * 1e3 / args.apply_reps
+    from meanas_amd.fdfd.autotune import tuned_slab
+    us64 = time_apply(op.plan('c64', slab=tuned_slab(op, _lib.MX_MIXED)), xa, ya, args.apply_reps, stream)
     bpc = op.apply_bytes_per_cell(8)
-    apply_bytes = bpc * op.ncell
-    achieved = apply_bytes / (apply_us * 1e-6) / 1e9
-    log(f'c64 apply: {apply_us:.2f} us, {bpc} B/cell, {achieved:.0f} GB/s ({plan.blocks} blocks)')
-    # complex128 apply for reference
-    xb = xa.to(torch.complex128)
-    yb = torch.empty_like(xb)
-    plan64 = op.plan('c128')
-    for _ in range(5):
-        plan64.apply(xb, yb, stream=stream)
-    e0.record(stream)
-    for _ in range(args.apply_reps // 2):
-        plan64.apply(xb, yb, stream=stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    apply64_us = e0.elapsed_time(e1) * 1e3 / (args.apply_reps // 2)
-    bpc64 = op.apply_bytes_per_cell(16)
-    extra['apply_c128'] = dict(us=apply64_us, bytes_per_cell=bpc64,
-                               gbs=bpc64 * op.ncell / (apply64_us * 1e-6) / 1e9)
+    extra['apply_c64'] = dict(us=us64, bytes_per_cell=bpc, gbs=bpc * op.ncell / (us64 * 1e-6) / 1e9)
+
+    # ---- fields after the solve: H = e2h(E) and the Poynting product, on the solved field
+    from meanas_amd.fdfd import functional
+    f_e2h = functional.e2h(w['omega'], w['dxes'])
+    f_exh = functional.poynting_e_cross_h(w['dxes'])
+    E_t = E_step.contiguous()
+    H_t = f_e2h(E_t)
+    post = {}
+    for name, fn, bpc in (('e2h', lambda: f_e2h(E_t), 96), ('poynting_e_cross_h', lambda: f_exh(E_t, H_t, True), 144)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(20):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / 20
+        post[name] = dict(us=round(us, 2), bytes=bpc * op.ncell, gbs=round(bpc * op.ncell / (us * 1e-6) / 1e9, 1))
+    log(f'post-processing: {post}')
+    extra['post_processing'] = post
 
     # ---- CPU baseline (oracle) on a bounded sample, rank 0 only
     cpu = None
@@ -371,10 +410,7 @@ def main():
         'converged': ok,
         'iterations': stats[-1]['iterations'], 'replacements': stats[-1]['replacements'],
         'rel_residual': stats[-1]['rel_residual'],
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'k_apply<float> (complex64 operator apply)', 'us_per_launch': apply_us,
-                     'bytes_per_cell': bpc, 'cells': op.ncell},
+        'roofline': roof(dom),
         'cpu_baseline': cpu,
     }
     line.update(extra)

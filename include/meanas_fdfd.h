@@ -155,6 +155,40 @@ int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const v
 int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys,
              const mx_options* opt, mx_stats* stats, void* stream);
 
+/* ---- fields after the solve ------------------------------------------------------
+ * Grid of a post-processing call: the 1D cell widths dxes[0] (E grid) and dxes[1]
+ * (H grid) as device arrays of complex128, lengths nx, ny, nz (fdmath/types.py:62-71). */
+typedef struct mx_grid {
+  int32_t nx, ny, nz, pad;
+  const void* dxe[3];
+  const void* dxh[3];
+} mx_grid;
+
+/* H from E: H = curl_forward(E) / (-i omega), times 1/mu, zero where pmc is set.
+ * Replaces fdfd/operators.py:253-282 (e2h, sparse) and fdfd/functional.py:89-117
+ * (e2h, matrix-free).  inv_mu: 3N complex128 (1/mu per component) or NULL; pmc: 3N bytes
+ * (nonzero = PMC) or NULL; e, h: 3N complex128 in vec() order. */
+int mx_e2h(const mx_grid* g, double omega_re, double omega_im, const void* inv_mu, const uint8_t* pmc,
+           const void* e, void* h, void* stream);
+
+/* The Poynting cross product with the E field shifted one cell forward, both fields
+ * weighted by their cell widths: s = E x H as fdfd/functional.py:187-221 computes it
+ * (poynting_e_cross_h), equal to poynting_e_cross(e, dxes) @ h of fdfd/operators.py:310-336.
+ * The time-averaged Poynting vector is Re(s)/2 with h replaced by conj(H); conj_h = 1 does
+ * that conjugation in the kernel.  e, h, s: 3N complex128. */
+int mx_poynting_e_cross_h(const mx_grid* g, const void* e, const void* h, int32_t conj_h, void* s, void* stream);
+
+/* Kernel timing of solver iterations (measurement; no counterpart in the reference).
+ * Sets up `sys` as mx_solve would (r = b - A x, p = r) with every stopping rule off, runs
+ * `reps` iterations on `stream` exactly as a solve launches them, with a HIP event
+ * between consecutive kernels, and reports the average microseconds per launch:
+ * us_out[0] operator apply (COCG epilogue), [1] its finishing kernel, [2] the vector
+ * update, [3] its finishing kernel, [4] the direction update of the classic form
+ * (a no-op launch for MX_FP64 systems in the fused form).  Fills min(n_out, 5) entries.
+ * x and the workspace are overwritten. */
+int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_options* opt, int32_t reps,
+                    double* us_out, int32_t n_out, void* stream);
+
 /* Continuous-batching session: systems are admitted into slots while others iterate;
  * no counterpart in the reference (its solves are one at a time).  A session owns its
  * device tables; vectors stay caller-owned. */

@@ -66,6 +66,12 @@ class MxOptions(ctypes.Structure):
     ]
 
 
+class MxGrid(ctypes.Structure):
+    """mx_grid: cell widths dxes[0] (E grid) and dxes[1] (H grid) as device complex128 arrays."""
+    _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32), ('pad', ctypes.c_int32),
+                ('dxe', c_void_p * 3), ('dxh', c_void_p * 3)]
+
+
 class MxStats(ctypes.Structure):
     _fields_ = [
         ('status', ctypes.c_int32), ('iterations', ctypes.c_int32),
@@ -95,6 +101,12 @@ PROTOTYPES = {
                               c_void_p, c_void_p, c_void_p]),
     'mx_solve': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.c_int32, ctypes.POINTER(MxSystem),
                                 ctypes.c_int32, ctypes.POINTER(MxOptions), ctypes.POINTER(MxStats), c_void_p]),
+    'mx_e2h': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p]),
+    'mx_poynting_e_cross_h': (ctypes.c_int, [ctypes.POINTER(MxGrid), c_void_p, c_void_p, ctypes.c_int32, c_void_p,
+                                             c_void_p]),
+    'mx_time_kernels': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), ctypes.POINTER(MxOptions),
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, c_void_p]),
     'mx_session_create': (c_void_p, [ctypes.c_int32, ctypes.POINTER(MxOptions)]),
     'mx_session_admit': (ctypes.c_int, [c_void_p, ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), c_void_p]),
     'mx_session_step': (ctypes.c_int, [c_void_p, ctypes.c_int32, c_void_p]),

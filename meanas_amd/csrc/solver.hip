@@ -9,6 +9,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "common.hpp"
@@ -371,6 +372,11 @@ mx_options defaults(const mx_options* opt) {
 }  // namespace
 
 // =====================================================================================
+namespace mx {
+// error reporting for the other translation units of the library (mx_last_error)
+int set_error(const std::string& msg) { return fail(msg); }
+}  // namespace mx
+
 extern "C" {
 
 int mx_abi_version(void) { return MX_ABI_VERSION; }
@@ -512,6 +518,81 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   if (e != hipSuccess) rc = fail(std::string("mx_residual: ") + hipGetErrorString(e));
   else if (norm2_out) *norm2_out = st.true_rnorm2;
   (void)hipFree(mem);
+  return rc;
+}
+
+int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_options* opt, int32_t reps,
+                    double* us_out, int32_t n_out, void* stream) {
+  if (!op || !sys || !us_out || n_out < 1 || reps < 1) return fail("mx_time_kernels: bad argument");
+  const mx_options o = defaults(opt);
+  if (o.mode != MX_MIXED && o.mode != MX_FP64) return fail("mode must be MX_MIXED or MX_FP64");
+  mx_system y = *sys;
+  y.op = 0;
+  Batch B;
+  if (build_batch(B, op, 1, &y, 1, o)) return -1;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const bool fp64 = o.mode == MX_FP64;
+  // set-up as in mx_solve, with every stopping and pausing rule off, so each kernel
+  // below runs its full work on every launch
+  SysState st;
+  std::memset(&st, 0, sizeof(st));
+  st.flags = SF_PAUSE | SF_RESTART | (fp64 ? SF_FUSED : 0);
+  st.maxiter = std::numeric_limits<int>::max();
+  SysState* dst = B.h_sys[0].st;
+  MX_TRY(hipMemcpyAsync(dst, &st, sizeof(st), hipMemcpyHostToDevice, s));
+  if (launch_zero_slot(fp64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) return fail("zero x_lo");
+  MX_TRY(launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+  MX_TRY(launch_fin(FIN_BNORM, B.d_sys, B.d_syslist, B.nsl, s));
+  if (enqueue_replacement(B, s)) return -1;
+  SysState snap;
+  MX_TRY(hipMemcpyAsync(&snap, dst, sizeof(snap), hipMemcpyDeviceToHost, s));
+  MX_TRY(hipStreamSynchronize(s));
+  if (snap.flags & (SF_DONE | SF_PAUSE | SF_MR)) return fail("mx_time_kernels: the system needs a nonzero, generic residual");
+  // The iteration runs as in a solve (each kernel right after the one it follows, so
+  // caches hold what they would), with an event between consecutive kernels: kernel k's
+  // time is the span from the event before it to the event after it.
+  constexpr int kKernels = 5;
+  auto launch = [&](int k) -> hipError_t {
+    switch (k) {
+      case 0: return fp64 ? launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s)
+                          : launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
+      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
+                          : launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
+      case 2: return launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s);
+      case 3: return fp64 ? launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s)
+                          : launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s);
+      default: return launch_fin(FIN_UPDATE, B.d_sys, B.d_syslist, B.nsl, s);
+    }
+  };
+  std::vector<hipEvent_t> ev((size_t)reps * (kKernels + 1), nullptr);
+  int rc = 0;
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  for (int w = 0; w < 2 && e == hipSuccess; ++w)
+    for (int k = 0; k < kKernels && e == hipSuccess; ++k) e = launch(k);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) {
+    hipEvent_t* er = &ev[(size_t)r * (kKernels + 1)];
+    e = hipEventRecord(er[0], s);
+    for (int k = 0; k < kKernels && e == hipSuccess; ++k) {
+      e = launch(k);
+      if (e == hipSuccess) e = hipEventRecord(er[k + 1], s);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  double tot[kKernels] = {0, 0, 0, 0, 0};
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    for (int k = 0; k < kKernels && e == hipSuccess; ++k) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev[(size_t)r * (kKernels + 1) + k], ev[(size_t)r * (kKernels + 1) + k + 1]);
+      tot[k] += ms;
+    }
+  if (e != hipSuccess) rc = fail(std::string("mx_time_kernels: ") + hipGetErrorString(e));
+  // output order: apply, its finishing kernel, update, its finishing kernel, direction update
+  const int order[kKernels] = {1, 2, 3, 4, 0};
+  for (int k = 0; k < std::min<int>(n_out, kKernels) && !rc; ++k) us_out[k] = 1e3 * tot[order[k]] / reps;
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
   return rc;
 }
 

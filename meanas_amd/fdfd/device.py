@@ -284,3 +284,25 @@ def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
     _lib.check(_lib.lib().mx_solve(descs, len(ops), sysarr, len(systems), ctypes.byref(opts), stats,
                                    torch_stream_handle(stream)), 'mx_solve')
     return [st.as_dict() for st in stats]
+
+
+KERNEL_NAMES = ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')
+
+
+def time_kernels(op: DeviceOperator, b: torch.Tensor, mode: int = _lib.MX_FP64, reps: int = 50,
+                 slab: int | None = None, stream=None) -> dict:
+    """Average microseconds per launch of each kernel of one solver iteration
+    (`mx_time_kernels`), timed with HIP events on `stream` on the system A x = b.
+    The tiling is the one a solve of this operator uses (autotuned slab unless given)."""
+    op._check_vec(b, torch.complex128)
+    x = torch.zeros_like(b)
+    work = torch.empty(op.work_bytes(mode), dtype=torch.uint8, device=op.device)
+    if slab is None:
+        from .autotune import tuned_slab
+        slab = tuned_slab(op, mode)
+    sysd = _lib.MxSystem(0, 1, int(slab), 0, 0.0, 0.0, b.data_ptr(), x.data_ptr(), work.data_ptr())
+    opts = _lib.MxOptions(mode, 0, 0, 0, 0, 1, 0.0, 0, 0)
+    us = (ctypes.c_double * len(KERNEL_NAMES))()
+    _lib.check(_lib.lib().mx_time_kernels(ctypes.byref(op.desc), ctypes.byref(sysd), ctypes.byref(opts), int(reps),
+                                          us, len(KERNEL_NAMES), torch_stream_handle(stream)), 'mx_time_kernels')
+    return dict(zip(KERNEL_NAMES, list(us)), slab=int(slab))

@@ -121,3 +121,23 @@ def test_residual_entry_point():
     want = b - reference_apply(omega, dxes, eps, None, None, None, x)
     assert rel(r.cpu().numpy(), want) < 1e-13
     assert abs(n2 - numpy.vdot(want, want).real) <= 1e-12 * n2
+
+
+@pytest.mark.gpu
+def test_time_kernels_reports_every_kernel():
+    """mx_time_kernels times each iteration kernel; the solve that follows still works."""
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, time_kernels, solve_batch
+    rng = numpy.random.default_rng(5)
+    shape = (12, 10, 70)
+    dxes = [[numpy.ones(n) for n in shape] for _ in range(2)]
+    eps = 1.0 + rng.random(3 * int(numpy.prod(shape)))
+    op = DeviceOperator(0.3 + 0.03j, dxes, eps)   # lossy: the solve below converges fast
+    b = torch.from_numpy(rng.standard_normal(op.n3) + 1j * rng.standard_normal(op.n3)).cuda()
+    for mode in (_lib.MX_FP64, _lib.MX_MIXED):
+        kt = time_kernels(op, b, mode, reps=5, slab=4)
+        for k in ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate'):
+            assert numpy.isfinite(kt[k]) and kt[k] > 0.0, kt
+    x = torch.zeros_like(b)
+    st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=1e-8)], mode=_lib.MX_FP64)[0]
+    assert st['status'] == 'converged', st

@@ -350,3 +350,29 @@ def test_reference_residual_and_poynting(case):
     dv = dxes[0][0][:, None, None] * dxes[0][1][None, :, None] * dxes[0][2][None, None, :]
     src = -((e * j.conj()).real * dv)[:, mask] / 2
     numpy.testing.assert_allclose(sum(planes), src.sum(), rtol=1e-6)
+
+
+def test_poynting_sparse_equals_functional_form():
+    """The oracle's sparse poynting_e_cross (operators.py:310-336) applied to h equals the
+    roll-based E x H of fdfd/functional.py:187-221, restated here directly; and e2h
+    equals curl_forward / (-i omega) assembled from scratch with numpy.roll."""
+    rng = numpy.random.default_rng(11)
+    shape = (5, 6, 7)
+    dxes = [[0.5 + rng.random(n) + 0.1j * rng.random(n) for n in shape] for _ in range(2)]
+    e = rng.standard_normal((3, *shape)) + 1j * rng.standard_normal((3, *shape))
+    h = rng.standard_normal((3, *shape)) + 1j * rng.standard_normal((3, *shape))
+    we = [dxes[0][0][:, None, None], dxes[0][1][None, :, None], dxes[0][2][None, None, :]]
+    wh = [dxes[1][0][:, None, None], dxes[1][1][None, :, None], dxes[1][2][None, None, :]]
+    ex, ey, ez = (e[c] * we[c] for c in range(3))
+    hx, hy, hz = (h[c] * wh[c] for c in range(3))
+    s = numpy.stack([numpy.roll(ey, -1, 0) * hz - numpy.roll(ez, -1, 0) * hy,
+                     numpy.roll(ez, -1, 1) * hx - numpy.roll(ex, -1, 1) * hz,
+                     numpy.roll(ex, -1, 2) * hy - numpy.roll(ey, -1, 2) * hx])
+    got = ref.poynting_e_cross(ref.vec(e), dxes) @ ref.vec(h)
+    numpy.testing.assert_allclose(got, ref.vec(s), rtol=1e-13, atol=1e-13)
+    # e2h from forward differences (f[n+1] - f[n]) / dx_e[n]
+    d = [lambda f, a=a: (numpy.roll(f, -1, a) - f) / we[a] for a in range(3)]
+    curl = numpy.stack([d[1](e[2]) - d[2](e[1]), d[2](e[0]) - d[0](e[2]), d[0](e[1]) - d[1](e[0])])
+    omega = 0.7 + 0.01j
+    numpy.testing.assert_allclose(ref.e2h(omega, dxes) @ ref.vec(e), ref.vec(curl / (-1j * omega)),
+                                  rtol=1e-12, atol=1e-12)
