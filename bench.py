@@ -379,6 +379,18 @@ def main():
     log(f'post-processing: {post}')
     extra['post_processing'] = post
 
+    # ---- the drop-in end to end: generic() from host numpy arrays to a host E (operator
+    # tables, host-device copies and the solve; not the metric, whose inputs are resident)
+    from meanas_amd.fdfd import solvers as gsolvers
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E_host = gsolvers.generic(w['omega'], w['dxes'], w['J'], w['epsilon'],
+                              matrix_solver_opts=dict(rtol=args.rtol))
+    t_e2e = time.perf_counter() - t0
+    extra['generic_end_to_end_s'] = t_e2e
+    log(f'generic() end to end: {t_e2e:.3f} s ({gsolvers.last_info})')
+    del E_host
+
     # ---- CPU baseline (oracle) on a bounded sample, rank 0 only
     cpu = None
     if not args.no_cpu_baseline:
