@@ -64,6 +64,8 @@ def parse():
                          'serve: Poisson request stream with continuous batching')
     ap.add_argument('--count', type=int, default=0, help='structures (uneven) or requests (serve)')
     ap.add_argument('--qps', type=float, default=4.0, help='serve: arrival rate')
+    ap.add_argument('--form', choices=['auto', 'fused', 'classic'], default='auto',
+                    help='COCG iteration form (auto: fused for fp64, classic for mixed)')
     ap.add_argument('--mode', choices=['mixed', 'fp64'], default='fp64',
                     help='solver of the timed step (the other one is the same-GPU comparison)')
     return ap.parse_args()
@@ -144,25 +146,26 @@ def batch_workloads(args):
     from meanas_amd.fdfd.device import DeviceOperator, solve_batch, time_kernels
     dev = torch.device('cuda:0')
     mode = _lib.MX_FP64 if args.mode == 'fp64' else _lib.MX_MIXED
+    fopt = dict(form=args.form if (mode == _lib.MX_FP64 or args.form != 'fused') else 'auto')
     if args.workload == 'multi_rhs':
         w = workloads.aniso_multi()
         op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
         bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).to(dev), coef=-1j * w['omega']) for J in w['Js']]
         xs = [torch.zeros_like(b) for b in bs]
         systems = [dict(op=0, b=b, x=x, rtol=args.rtol, maxiter=200000) for b, x in zip(bs, xs)]
-        solve_batch([op], systems[:1], mode=mode)            # warm-up (autotune, code load)
+        solve_batch([op], systems[:1], mode=mode, options=fopt)   # warm-up (autotune, code load)
         for x in xs:
             x.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = solve_batch([op], systems, mode=mode)
+        st = solve_batch([op], systems, mode=mode, options=fopt)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         line = dict(metric='solves/s, 16 right-hand sides on one structure', value=len(xs) / wall, unit='solves/s',
                     wall_s=wall, converged=sum(s['status'] == 'converged' for s in st),
                     iterations=[s['iterations'] for s in st],
                     config={'workload': 'aniso_multi: 128x96x96, anisotropic box eps, 16 Gaussian sheets', 'rtol': args.rtol,
-                            'mode': args.mode})
+                            'mode': args.mode, 'form': args.form})
     elif args.workload == 'uneven':
         probs = workloads.uneven_batch(count=args.count or 32)
         ops, systems = [], []
@@ -176,7 +179,7 @@ def batch_workloads(args):
             tuned_slab(op, mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = solve_batch(ops, systems, mode=mode)
+        st = solve_batch(ops, systems, mode=mode, options=fopt)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         cells = sum(op.ncell for op in ops)
@@ -186,14 +189,14 @@ def batch_workloads(args):
                     systems=[dict(shape=op.shape, omega=round(w['omega'], 5), status=x['status'], it=x['iterations'],
                                   res=x['rel_residual']) for op, w, x in zip(ops, probs, st)],
                     config={'workload': f'uneven_batch: {len(ops)} structures, edges 48-128, 5% PEC', 'rtol': args.rtol,
-                            'mode': args.mode})
+                            'mode': args.mode, 'form': args.form})
     else:
         from meanas_amd.serve import Request, Server, summarize
         reqs = []
         for k, r in enumerate(workloads.serving_stream(args.count or 40, args.qps)):
             reqs.append(Request(omega=r['omega'], dxes=r['dxes'], J=r['J'], epsilon=r['epsilon'], adjoint=r['adjoint'],
                                 rtol=args.rtol, arrival=r['arrival'], tag=dict(k=k, adjoint=r['adjoint'], shape=r['shape'])))
-        srv = Server(capacity=16, mode=mode, chunk=16)
+        srv = Server(capacity=16, mode=mode, chunk=16, options=fopt)
         # warm the autotuner cache for every shape outside the timed run
         from meanas_amd.fdfd.autotune import tuned_slab
         for r in reqs:
@@ -208,7 +211,7 @@ def batch_workloads(args):
                     unit='solves/s', latency_ms={'p50': summ['p50_ms'], 'p99': summ['p99_ms']}, **summ,
                     config={'workload': f'serving: {len(reqs)} Poisson requests at {args.qps} QPS, edges lognormal '
                                         'in [48,256], 30% adjoint, anisotropic box eps', 'rtol': args.rtol,
-                            'mode': args.mode, 'capacity': 16})
+                            'mode': args.mode, 'form': args.form, 'capacity': 16})
     line.update(n_gpus=1, dtype='c128' if args.mode == 'fp64' else 'c64 inner + f64 residual', data='synthetic')
     print(json.dumps(line), flush=True)
 
@@ -268,8 +271,9 @@ def main():
 
     def solve(mode, rtol, xout):
         xout.zero_()
+        form = args.form if (mode == _lib.MX_FP64 or args.form != 'fused') else 'auto'
         return solve_batch([op], [dict(op=0, b=b, x=xout, rtol=rtol, atol=0.0, maxiter=maxiter, work=work[mode])],
-                           mode=mode)[0]
+                           mode=mode, options=dict(form=form))[0]
 
     step_mode = _lib.MX_MIXED if args.mode == 'mixed' else _lib.MX_FP64
     # ---- the measured step: complete solves

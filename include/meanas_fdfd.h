@@ -99,7 +99,18 @@ typedef struct mx_options {
   int32_t stall_limit; /* stop with MX_STALLED after this many consecutive replacements at which the
                           recursive residual had drifted >= 100x below the true one (0 = automatic: 3;
                           < 0 = never) */
+  int32_t form;        /* iteration form: MX_FORM_AUTO (fused for MX_FP64, classic for MX_MIXED),
+                          MX_FORM_FUSED (MX_FP64 only) or MX_FORM_CLASSIC */
+  int32_t reserved;    /* 0 */
 } mx_options;
+
+/* COCG iteration forms.  Fused: the direction update rides in the x/r pass with beta
+ * predicted from q^T q (9 vector streams per iteration); a system whose recursion loses
+ * track of its true residual falls back to the classic form by itself.  Classic: beta
+ * from the exact r^T r, the direction updated in its own pass (11 streams). */
+#define MX_FORM_AUTO 0
+#define MX_FORM_FUSED 1
+#define MX_FORM_CLASSIC 2
 
 typedef struct mx_stats {
   int32_t status;        /* MX_CONVERGED, ... */

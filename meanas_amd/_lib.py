@@ -25,6 +25,8 @@ MX_MAXITER = 1
 MX_BREAKDOWN = 2
 MX_NONFINITE = 3
 MX_STALLED = 4
+MX_FORM_AUTO, MX_FORM_FUSED, MX_FORM_CLASSIC = 0, 1, 2
+FORMS = {'auto': MX_FORM_AUTO, 'fused': MX_FORM_FUSED, 'classic': MX_FORM_CLASSIC}
 STATUS_NAMES = {0: 'converged', 1: 'maxiter', 2: 'breakdown', 3: 'nonfinite', 4: 'stalled'}
 
 MX_MAP_RHS = 0
@@ -63,6 +65,7 @@ class MxOptions(ctypes.Structure):
         ('chunk', ctypes.c_int32), ('check_every', ctypes.c_int32), ('use_graph', ctypes.c_int32),
         ('delta', ctypes.c_double),
         ('stall_iters', ctypes.c_int32), ('stall_limit', ctypes.c_int32),
+        ('form', ctypes.c_int32), ('reserved', ctypes.c_int32),
     ]
 
 

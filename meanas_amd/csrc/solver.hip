@@ -354,6 +354,13 @@ void fill_stats(const SysState& st, mx_stats* out) {
   else out->status = MX_MAXITER;
 }
 
+bool valid_options(const mx_options& o, std::string* why) {
+  if (o.mode != MX_MIXED && o.mode != MX_FP64) { *why = "mode must be MX_MIXED or MX_FP64"; return false; }
+  if (o.form != MX_FORM_FUSED && o.form != MX_FORM_CLASSIC) { *why = "form must be MX_FORM_AUTO, MX_FORM_FUSED or MX_FORM_CLASSIC"; return false; }
+  if (o.form == MX_FORM_FUSED && o.mode != MX_FP64) { *why = "the fused form needs complex128 inner vectors (MX_FP64)"; return false; }
+  return true;
+}
+
 mx_options defaults(const mx_options* opt) {
   mx_options o;
   std::memset(&o, 0, sizeof(o));
@@ -366,6 +373,7 @@ mx_options defaults(const mx_options* opt) {
   // iterations, so FP64 runs to maxiter unless the caller sets a window.
   if (o.stall_iters == 0) o.stall_iters = (o.mode == MX_MIXED) ? kStallWindow : -1;
   if (o.stall_limit == 0) o.stall_limit = 3;
+  if (o.form == MX_FORM_AUTO) o.form = (o.mode == MX_FP64) ? MX_FORM_FUSED : MX_FORM_CLASSIC;
   return o;
 }
 
@@ -525,7 +533,8 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
                     double* us_out, int32_t n_out, void* stream) {
   if (!op || !sys || !us_out || n_out < 1 || reps < 1) return fail("mx_time_kernels: bad argument");
   const mx_options o = defaults(opt);
-  if (o.mode != MX_MIXED && o.mode != MX_FP64) return fail("mode must be MX_MIXED or MX_FP64");
+  std::string why;
+  if (!valid_options(o, &why)) return fail(why);
   mx_system y = *sys;
   y.op = 0;
   Batch B;
@@ -536,7 +545,7 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
   // below runs its full work on every launch
   SysState st;
   std::memset(&st, 0, sizeof(st));
-  st.flags = SF_PAUSE | SF_RESTART | (fp64 ? SF_FUSED : 0);
+  st.flags = SF_PAUSE | SF_RESTART | (o.form == MX_FORM_FUSED ? SF_FUSED : 0);
   st.maxiter = std::numeric_limits<int>::max();
   SysState* dst = B.h_sys[0].st;
   MX_TRY(hipMemcpyAsync(dst, &st, sizeof(st), hipMemcpyHostToDevice, s));
@@ -600,7 +609,8 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
              mx_stats* stats, void* stream) {
   if (!ops || nops < 1 || !systems || nsys < 1) return fail("mx_solve: empty batch");
   const mx_options o = defaults(opt);
-  if (o.mode != MX_MIXED && o.mode != MX_FP64) return fail("mode must be MX_MIXED or MX_FP64");
+  std::string why;
+  if (!valid_options(o, &why)) return fail(why);
   Batch B;
   if (build_batch(B, ops, nops, systems, nsys, o)) return -1;
   hipStream_t user = reinterpret_cast<hipStream_t>(stream);
@@ -615,7 +625,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   for (int i = 0; i < nsys; ++i) {
     SysState& st = init[i];
     std::memset(&st, 0, sizeof(st));
-    st.flags = SF_PAUSE | SF_RESTART | (o.mode == MX_FP64 ? SF_FUSED : 0);
+    st.flags = SF_PAUSE | SF_RESTART | (o.form == MX_FORM_FUSED ? SF_FUSED : 0);
     st.rtol = systems[i].rtol;
     st.atol = systems[i].atol;
     st.maxiter = std::max(1, systems[i].maxiter);
@@ -823,6 +833,10 @@ mx_session* mx_session_create(int32_t capacity, const mx_options* opt) {
   mx_session* S = new (std::nothrow) mx_session();
   if (!S) { fail("out of host memory"); return nullptr; }
   S->o = defaults(opt);
+  {
+    std::string why;
+    if (!valid_options(S->o, &why)) { fail(why); delete S; return nullptr; }
+  }
   S->capacity = capacity;
   S->ty = S->o.tile_rows ? S->o.tile_rows : kAutoTileRows;
   S->h_sys.assign(capacity, SysDesc{});
@@ -886,7 +900,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   MX_TRY(hipStreamSynchronize(s));
   SysState* st = &S->h_state[slot];
   std::memset(st, 0, sizeof(*st));
-  st->flags = SF_PAUSE | SF_RESTART | (S->o.mode == MX_FP64 ? SF_FUSED : 0);
+  st->flags = SF_PAUSE | SF_RESTART | (S->o.form == MX_FORM_FUSED ? SF_FUSED : 0);
   st->rtol = y->rtol;
   st->atol = y->atol;
   st->maxiter = std::max(1, y->maxiter);

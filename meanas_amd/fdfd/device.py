@@ -258,7 +258,7 @@ def solve_batch(ops: Sequence[DeviceOperator], systems: Sequence[dict], mode: in
 
 def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
     o = dict(tile_rows=0, slab=0, chunk=0, check_every=0, use_graph=1, delta=0.0, stall_iters=0, stall_limit=0,
-             autotune=True)
+             autotune=True, form='auto')
     if options:
         o.update(options)
     descs = (_lib.MxOperator * len(ops))(*[op.desc for op in ops])
@@ -279,7 +279,7 @@ def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
         sysarr[i] = _lib.MxSystem(s['op'], int(s.get('maxiter', 10 * op.n3)), slab, 0, float(s.get('rtol', 1e-5)),
                                   float(s.get('atol', 0.0)), s['b'].data_ptr(), s['x'].data_ptr(), work.data_ptr())
     opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'], o['delta'],
-                          o['stall_iters'], o['stall_limit'])
+                          o['stall_iters'], o['stall_limit'], _lib.FORMS[o['form']])
     stats = (_lib.MxStats * len(systems))()
     _lib.check(_lib.lib().mx_solve(descs, len(ops), sysarr, len(systems), ctypes.byref(opts), stats,
                                    torch_stream_handle(stream)), 'mx_solve')

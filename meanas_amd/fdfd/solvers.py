@@ -20,6 +20,8 @@ reference accepts, cannot be honoured by a matrix-free solver and raises TypeErr
 matrix_solver_opts honours scipy qmr's keys: rtol (default 1e-5), atol (default 0),
 maxiter (default 10 * 3N) and x0 (scaled initial guess); as in the reference, passing
 E_guess writes the scaled guess into matrix_solver_opts['x0'] (solvers.py:127-132).
+One key of this build: form ('auto', 'fused' or 'classic', the COCG iteration form of
+mx_options; 'auto' is fused for FP64).
 Convergence means ||b - A x|| <= max(rtol ||b||, atol) for the FP64 true residual of
 the scaled system.  Like the reference, a solve that does not converge still returns
 an iterate -- here the best one (lowest FP64 true residual seen), where the reference
@@ -78,7 +80,7 @@ def generic(omega: complex, dxes, J, epsilon, mu=None, *, pec=None, pmc=None, ad
     if matrix_solver_opts is None:
         matrix_solver_opts = dict()
     mode = _method(matrix_solver)
-    unknown = set(matrix_solver_opts) - {'rtol', 'atol', 'maxiter', 'x0', 'callback', 'tol'}
+    unknown = set(matrix_solver_opts) - {'rtol', 'atol', 'maxiter', 'x0', 'callback', 'tol', 'form'}
     if unknown:
         raise TypeError(f'unsupported matrix_solver_opts keys: {sorted(unknown)}')
     shape = grid_shape(dxes)
@@ -123,7 +125,13 @@ def generic(omega: complex, dxes, J, epsilon, mu=None, *, pec=None, pmc=None, ad
         last_info = dict(status='converged', iterations=0, replacements=0, rel_residual=0.0, bnorm=0.0)
         return numpy.zeros(n3, dtype=numpy.complex128)
 
-    stats = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol, atol=atol, maxiter=maxiter)], mode=mode)[0]
+    form = matrix_solver_opts.get('form', 'auto')
+    if form not in _lib.FORMS:
+        raise ValueError(f"matrix_solver_opts['form'] must be one of {sorted(_lib.FORMS)}")
+    if mode == _lib.MX_MIXED and form == 'fused':
+        form = 'auto'   # complex64 inner vectors always use the classic form
+    stats = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol, atol=atol, maxiter=maxiter)], mode=mode,
+                        options=dict(form=form))[0]
     E = op.map(_lib.MX_MAP_E_ADJ if adjoint else _lib.MX_MAP_E, x)
     last_info = dict(stats, method='cocg_fp64' if mode == _lib.MX_FP64 else 'cocg_mixed', adjoint=adjoint)
     if fp64_reason:

@@ -93,11 +93,11 @@ class Server:
             hbm_budget = int(0.8 * free)
         self.hbm_budget = hbm_budget
         o = dict(tile_rows=0, slab=0, chunk=chunk, check_every=0, use_graph=1, delta=0.0,
-                 stall_iters=0, stall_limit=0)
+                 stall_iters=0, stall_limit=0, form='auto')
         if options:
             o.update(options)
         self.opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'],
-                                   o['delta'], o['stall_iters'], o['stall_limit'])
+                                   o['delta'], o['stall_iters'], o['stall_limit'], _lib.FORMS[o['form']])
         L = _lib.lib()
         self.handle = L.mx_session_create(capacity, ctypes.byref(self.opts))
         if not self.handle:

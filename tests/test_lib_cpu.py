@@ -37,7 +37,7 @@ def test_struct_sizes_match_header_layout():
     # mx_operator: 4 int32 + 2 double + 7 pointers ; mx_system: 4 int32 + 2 double + 3 pointers
     assert ctypes.sizeof(_lib.MxOperator) == 16 + 16 + 7 * 8
     assert ctypes.sizeof(_lib.MxSystem) == 16 + 16 + 3 * 8
-    assert ctypes.sizeof(_lib.MxOptions) == 6 * 4 + 8 + 2 * 4
+    assert ctypes.sizeof(_lib.MxOptions) == 6 * 4 + 8 + 4 * 4
     assert ctypes.sizeof(_lib.MxStats) == 16 + 16
     assert ctypes.sizeof(_lib.MxGrid) == 16 + 6 * 8
 
