@@ -54,6 +54,9 @@ def parse():
     ap.add_argument('--cpu-iters', type=int, default=40)
     ap.add_argument('--apply-sweep', action='store_true',
                     help='only time the operator apply over tile heights and slab depths')
+    ap.add_argument('--iter-sweep', action='store_true',
+                    help='only time the kernels of the FP64 iteration, in solve order, over tile heights and '
+                         'slab depths')
     ap.add_argument('--apply-only', action='store_true',
                     help='only run the operator apply (complex64 and complex128, default tiling) and the '
                          'kernels of one FP64 solver iteration: the program the hardware-counter profiles '
@@ -238,6 +241,16 @@ def main():
     op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
     if args.apply_sweep:
         apply_sweep(op, dev)
+        return
+    if args.iter_sweep:
+        # kernels of the FP64 iteration in solve order (caches as in a solve) over tilings
+        J = torch.from_numpy(w['J']).to(dev)
+        b = op.map(_lib.MX_MAP_RHS, J, coef=-1j * w['omega'])
+        for ty in (2, 4, 8):
+            for slab in (4, 8, 16, 32):
+                kt = time_kernels(op, b, _lib.MX_FP64, reps=30, slab=slab, tile_rows=ty)
+                print(json.dumps(dict(tile_rows=ty, slab=slab, **{k: round(v, 2) for k, v in kt.items() if k != 'slab'})),
+                      flush=True)
         return
     if args.apply_only:
         stream = torch.cuda.current_stream()

@@ -14,6 +14,7 @@
 // a one-cell halo on the minus side goes to LDS, and the output is formed from H(i), its
 // y-/z- neighbours in LDS and H(i-1) kept in registers.  z is the contiguous axis: a wave
 // reads 64 consecutive complex values per row.
+#include <cstdlib>
 #include <type_traits>
 
 #include "common.hpp"
@@ -67,7 +68,7 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 // and holding it to 3 waves (168) spills, so it runs 2.
 template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : 2;
 
-template <typename T, int TY, int EPI, typename RT>
+template <typename T, int TY, int EPI, typename RT, int PD>
 __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpView<T>* __restrict__ ops,
                                                                  const SysDesc* __restrict__ systems,
                                                                  const BlockMap* __restrict__ bmap,
@@ -173,9 +174,11 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     if (has_row) put(row_lds, R.r, R.mr);
     if (has_col) put(col_lds, R.c, R.mc);
   };
-  // Register prefetch: the loads of plane i+1 are in flight from the middle of step i-1
-  // until the H phase of step i.
-  Regs R;
+  // Register prefetch, PD register sets used in turn: the loads of plane i+1 are issued
+  // in the middle of step i-PD and consumed by the H phase of step i.
+  static_assert(PD == 1 || PD == 2, "prefetch depth 1 or 2");
+  Regs RA, RB;
+  Regs& R = RA;
 
   // per-thread y/z coefficients (wrapped indices)
   const FaFb<T> cyj = coef_u(cy, jw), czk = coef_at(cz, kw);
@@ -213,7 +216,9 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     hpz = hscale(hz, 2, m, cell);
     __syncthreads();
     commit(R);                      // plane ib
-    fetch(R, wrapi(ib + 1, nx));    // plane ib+1 (H(ie-1) needs plane ie, and ib < ie)
+    fetch(RA, wrapi(ib + 1, nx));   // plane ib+1 (H(ie-1) needs plane ie, and ib < ie)
+    if constexpr (PD == 2)
+      if (ib + 2 <= ie) fetch(RB, wrapi(ib + 2, nx));
   }
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -224,8 +229,8 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
   RT* __restrict__ pdir = reinterpret_cast<RT*>(S.vec[VS_P]);
 
-#pragma unroll 1
-  for (int i = ib; i < ie; ++i) {
+  // step i: Rn holds plane i+1
+  auto step = [&](const int i, Regs& Rn) {
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
@@ -246,8 +251,8 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     auto h_phase = [&](auto RXt, auto RYt) {
       constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
       const cplx<T> hx = dif<RY>(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif<false>(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
-      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, pec(R.o, R.mo, 2), cxi.fb, ec[2]);
-      const cplx<T> hz = dif<RX>(cxi.fa, pec(R.o, R.mo, 1), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
+      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, pec(Rn.o, Rn.mo, 2), cxi.fb, ec[2]);
+      const cplx<T> hz = dif<RX>(cxi.fa, pec(Rn.o, Rn.mo, 1), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
       sH[0][r][q] = hscale(hx, 0, mown, cell);
       sH[1][r][q] = hscale(hy, 1, mown, cell);
       sH[2][r][q] = hscale(hz, 2, mown, cell);
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
         const uint8_t mh = sM[0][q];
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
         const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
-        const cplx<T> gz = dif<RX>(cxi.fa, pec(R.r, R.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
+        const cplx<T> gz = dif<RX>(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[0][0][q] = hscale(gx, 0, mh, hcell);
         sH[2][0][q] = hscale(gz, 2, mh, hcell);
       }
@@ -266,7 +271,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
         const uint8_t mh = sM[rr][0];
         const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
-        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(R.c, R.mc, 2), cxi.fb, e_(2, rr, 0));
+        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
         sH[1][rr][0] = hscale(gy, 1, mh, hcell);
       }
@@ -280,8 +285,8 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     }
     __syncthreads();   // H(i) is in LDS; every read of plane i is done
     if (i + 1 < ie) {
-      commit(R);                          // plane i+1 replaces plane i
-      fetch(R, wrapi(i + 2, nx));         // plane i+2
+      commit(Rn);                                         // plane i+1 replaces plane i
+      if (i + 1 + PD <= ie) fetch(Rn, wrapi(i + 1 + PD, nx));   // plane i+1+PD into the freed set
     }
 
     const cplx<T> hx0 = sH[0][r][q], hy0 = sH[1][r][q], hz0 = sH[2][r][q];
@@ -350,6 +355,16 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
         }
       }
     }
+  };
+  if constexpr (PD == 1) {
+#pragma unroll 1
+    for (int i = ib; i < ie; ++i) step(i, RA);
+  } else {
+#pragma unroll 1
+    for (int i = ib; i < ie; i += 2) {
+      step(i, RA);
+      if (i + 1 < ie) step(i + 1, RB);
+    }
   }
 
   if constexpr (EPI == EPI_COCG) {
@@ -363,16 +378,38 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
 }
 
 // ---- host-side launch helpers ------------------------------------------------------
+// Prefetch depth (register sets of planes in flight): MEANAS_AMD_PREFETCH=1|2 overrides
+// the default for tuning runs.
+static int prefetch_depth(size_t elem) {
+  static const int forced = [] {
+    const char* v = getenv("MEANAS_AMD_PREFETCH");
+    return v ? atoi(v) : 0;
+  }();
+  if (forced == 1 || forced == 2) return forced;
+  (void)elem;
+  return 1;   // depth 2 (complex128 only: complex64 would spill) measured no faster in a solve
+}
+
 template <typename T, int EPI, typename RT>
 static hipError_t launch_apply_ty(int ty, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
                                   const void* in_ptr = nullptr, void* out_ptr = nullptr) {
-  switch (ty) {
-    case 2: hipLaunchKernelGGL((k_apply<T, 2, EPI, RT>), dim3(nblocks), dim3(64, 2), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 4: hipLaunchKernelGGL((k_apply<T, 4, EPI, RT>), dim3(nblocks), dim3(64, 4), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
-    case 8: hipLaunchKernelGGL((k_apply<T, 8, EPI, RT>), dim3(nblocks), dim3(64, 8), 0, s, ops, sys, bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr); break;
+  const int pd = prefetch_depth(sizeof(T));
+#define MX_APPLY_CASE(TY_, PD_)                                                                              \
+  case TY_ * 10 + PD_:                                                                                     \
+    hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys, bmap, \
+                       in_slot, out_slot, skip, need, in_ptr, out_ptr);                                    \
+    break;
+  switch (ty * 10 + pd) {
+    MX_APPLY_CASE(2, 1)
+    MX_APPLY_CASE(4, 1)
+    MX_APPLY_CASE(8, 1)
+    MX_APPLY_CASE(2, 2)
+    MX_APPLY_CASE(4, 2)
+    MX_APPLY_CASE(8, 2)
     default: return hipErrorInvalidValue;
   }
+#undef MX_APPLY_CASE
   return hipGetLastError();
 }
 

@@ -290,7 +290,7 @@ KERNEL_NAMES = ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')
 
 
 def time_kernels(op: DeviceOperator, b: torch.Tensor, mode: int = _lib.MX_FP64, reps: int = 50,
-                 slab: int | None = None, stream=None) -> dict:
+                 slab: int | None = None, stream=None, tile_rows: int = 0, form: str = 'auto') -> dict:
     """Average microseconds per launch of each kernel of one solver iteration
     (`mx_time_kernels`), timed with HIP events on `stream` on the system A x = b.
     The tiling is the one a solve of this operator uses (autotuned slab unless given)."""
@@ -301,7 +301,7 @@ def time_kernels(op: DeviceOperator, b: torch.Tensor, mode: int = _lib.MX_FP64, 
         from .autotune import tuned_slab
         slab = tuned_slab(op, mode)
     sysd = _lib.MxSystem(0, 1, int(slab), 0, 0.0, 0.0, b.data_ptr(), x.data_ptr(), work.data_ptr())
-    opts = _lib.MxOptions(mode, 0, 0, 0, 0, 1, 0.0, 0, 0)
+    opts = _lib.MxOptions(mode, int(tile_rows), 0, 0, 0, 1, 0.0, 0, 0, _lib.FORMS[form])
     us = (ctypes.c_double * len(KERNEL_NAMES))()
     _lib.check(_lib.lib().mx_time_kernels(ctypes.byref(op.desc), ctypes.byref(sysd), ctypes.byref(opts), int(reps),
                                           us, len(KERNEL_NAMES), torch_stream_handle(stream)), 'mx_time_kernels')
