@@ -15,6 +15,9 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdfd/operators.py:310-336        poynting_e_cross
   meanas/fdfd/scpml.py:18-38,98-167       prepare_s_function / stretch_with_scpml
   meanas/fdfd/solvers.py:21-141           _scipy_qmr / generic
+  meanas/fdmath/operators.py:55-99        shift_with_mirror
+  meanas/fdfd/operators.py:366-450        e_tfsf_source / e_boundary_source
+  meanas/fdfd/waveguide_3d.py:113-156,206-252  compute_source / expand_e
 Third-party arithmetic: scipy.sparse (CSR products) and scipy.sparse.linalg.qmr
 (reference pins scipy~=1.14, pyproject.toml:44; this container has 1.15.3).
 
@@ -37,7 +40,13 @@ How each function is pinned (tests/test_oracle.py):
                        system); the restated reference tests test_residual and
                        test_poynting_planes (meanas/test/test_fdfd.py:14-55); the 1D
                        discrete Green's function through a PML-terminated line.
-  e2h / poynting    -- pinned through test_poynting_planes (energy balance).
+  e2h / poynting    -- pinned through test_poynting_planes (energy balance), and
+                       against roll-based restatements of fdfd/functional.py:89-117,187-221.
+  shift_with_mirror -- explicit index table on a small grid (mirror at both edges).
+  sources           -- e_tfsf_source, e_boundary_source, expand_e and compute_source
+                       pinned together by the restated one-way PML test
+                       (meanas/test/test_fdfd_pml.py:14-28): the source launches a wave
+                       in one direction only (reverse |E|^2 < 1e-12, forward 1 +- 3e-6).
 Deviation, by necessity: the adjoint branch uses `.conj().T` where the reference
 writes `.H`, an attribute scipy sparse arrays do not have (solvers.py:121).
 """
@@ -201,6 +210,81 @@ def poynting_e_cross(e, dxes) -> sparse.csr_array:
               [dg(fy @ ez), None, dg(fy @ -ex)],
               [dg(fz @ -ey), dg(fz @ ex), None]]
     return (sparse.block_array(blocks) @ dg(numpy.concatenate(dxb))).tocsr()
+
+
+def shift_with_mirror(axis: int, shape: Sequence[int], shift_distance: int = 1) -> sparse.csr_array:
+    """Shift by `shift_distance` along `axis`, mirroring past the edges
+    (fdmath/operators.py:55-99): (S f)[n] = f[m(n + d)], m reflecting at both ends."""
+    if shift_distance >= shape[axis]:
+        raise ValueError('shift too large for the axis')
+
+    def mirrored(n: int, d: int) -> numpy.ndarray:
+        v = numpy.arange(n) + d
+        v = numpy.where(v >= n, 2 * n - v - 1, v)
+        return numpy.where(v < 0, -1 - v, v)
+
+    idx = [mirrored(n, shift_distance if a == axis else 0) for a, n in enumerate(shape)]
+    cols = numpy.ravel_multi_index(numpy.meshgrid(*idx, indexing='ij'), shape).ravel()
+    n = int(numpy.prod(shape))
+    return sparse.csr_array((numpy.ones(n), (numpy.arange(n), cols)), shape=(n, n))
+
+
+def e_tfsf_source(TF_region, omega: complex, dxes, epsilon, mu=None) -> sparse.csr_array:
+    """(A Q - Q A) / (-i omega), A = e_full, Q = diag(TF_region) (fdfd/operators.py:366-394)."""
+    A = e_full(omega, dxes, epsilon, mu)
+    Q = sparse.diags_array(numpy.asarray(TF_region, dtype=float))
+    return ((A @ Q - Q @ A) / (-1j * omega)).tocsr()
+
+
+def e_boundary_source(mask, omega: complex, dxes, epsilon, mu=None,
+                      periodic_mask_edges: bool = False) -> sparse.csr_array:
+    """e_tfsf_source restricted to the cells where a one-cell shift of the mask (along
+    any axis longer than 1, either way; mirror or periodic edges) changes it
+    (fdfd/operators.py:397-450)."""
+    full = e_tfsf_source(mask, omega, dxes, epsilon, mu)
+    shape = _grid_shape(dxes[0])
+    n = int(numpy.prod(shape))
+    m = numpy.asarray(mask, dtype=float)
+    jmask = numpy.zeros(m.shape, dtype=bool)
+    for axis in range(3):
+        if shape[axis] == 1:
+            continue
+        for pol in (-1, 1):
+            s = shift_circ(axis, shape, pol) if periodic_mask_edges else shift_with_mirror(axis, shape, pol)
+            r = s - sparse.eye_array(n)
+            jmask |= numpy.abs(sparse.block_diag((r, r, r)) @ m) != 0
+    return (sparse.diags_array(jmask.astype(float)) @ full).tocsr()
+
+
+def expand_e(E, wavenumber: complex, dxes, axis: int, polarity: int, slices) -> numpy.ndarray:
+    """Copy the slice `slices` of E along `axis` with the propagation phase
+    exp(-i polarity k r), r the E-grid position relative to the slice
+    (fdfd/waveguide_3d.py:206-252)."""
+    slices = tuple(slices)
+    a_shape = numpy.roll([1, -1, 1, 1], axis)
+    pos = numpy.real(dxes[0][axis]).cumsum()
+    r = pos - pos[slices[axis]]
+    phase = numpy.exp(polarity * -1j * wavenumber * r).reshape(a_shape)
+    out = numpy.zeros_like(E)
+    full = list(slices)
+    full[axis] = slice(E.shape[axis + 1])
+    out[(slice(None), *full)] = phase * numpy.asarray(E)[(slice(None), *slices)]
+    return out
+
+
+def compute_source(E, wavenumber: complex, omega: complex, dxes, axis: int, polarity: int, slices,
+                   epsilon, mu=None) -> numpy.ndarray:
+    """One-way source: e_boundary_source of the half-space mask starting at the slice,
+    applied to the expanded field (fdfd/waveguide_3d.py:113-156)."""
+    Ex = expand_e(E, wavenumber, dxes, axis, polarity, slices)
+    smask = [slice(None)] * 4
+    smask[axis + 1] = slice(slices[axis].start, None) if polarity > 0 else slice(None, slices[axis].stop)
+    mask = numpy.zeros_like(Ex, dtype=int)
+    mask[tuple(smask)] = 1
+    eps_v = vec(epsilon)
+    mu_v = None if mu is None else vec(mu)
+    J = e_boundary_source(vec(mask), omega, dxes, eps_v, mu_v) @ vec(Ex)
+    return unvec(J, E.shape[1:])
 
 
 # ---------------------------------------------------------------------------

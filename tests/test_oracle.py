@@ -376,3 +376,59 @@ def test_poynting_sparse_equals_functional_form():
     omega = 0.7 + 0.01j
     numpy.testing.assert_allclose(ref.e2h(omega, dxes) @ ref.vec(e), ref.vec(curl / (-1j * omega)),
                                   rtol=1e-12, atol=1e-12)
+
+
+def test_shift_with_mirror_index_table():
+    """fdmath/operators.py:55-99 on a 4-cell axis: +1 -> [1,2,3,3], -1 -> [0,0,1,2]."""
+    f = numpy.arange(4.0)
+    numpy.testing.assert_array_equal(ref.shift_with_mirror(0, (4, 1, 1), 1) @ f, [1, 2, 3, 3])
+    numpy.testing.assert_array_equal(ref.shift_with_mirror(0, (4, 1, 1), -1) @ f, [0, 0, 1, 2])
+    g = numpy.arange(12.0).reshape(3, 4)[None].reshape(-1)          # shape (1, 3, 4), z fastest
+    numpy.testing.assert_array_equal((ref.shift_with_mirror(2, (1, 3, 4), 1) @ g).reshape(3, 4)[0], [1, 2, 3, 3])
+
+
+def _pml_case(shape3, polarity, eps_fg, dx, solve):
+    """meanas/test/test_fdfd_pml.py:14-28,73-184 with `solve` in place of generic."""
+    shape = (3, *shape3)
+    omega = 1 / 1500
+    epsilon = numpy.full(shape, eps_fg)
+    dxes = [[numpy.full(s, dx) for s in shape3] for _ in range(2)]
+    dim = int(numpy.where(numpy.array(shape3) > 1)[0][0])
+    for pol in (-1, 1):
+        dxes = ref.stretch_with_scpml(dxes, axis=dim, polarity=pol, omega=omega, epsilon_effective=eps_fg,
+                                      thickness=10)
+    other = [0, 1, 2]
+    other.remove(dim)
+    mid = shape3[dim] // 2
+    dx_prop = (dxes[0][dim][mid] + dxes[1][dim][mid]) / 2
+    center = numpy.zeros(shape, dtype=bool)
+    center[other, shape3[0] // 2, shape3[1] // 2, shape3[2] // 2] = True
+    k = omega * numpy.sqrt(epsilon[center].mean())
+    k_corr = 2 / dx_prop * numpy.arcsin(k * dx_prop / 2)
+    e = numpy.zeros(shape, dtype=complex)
+    e[center] = 1 / numpy.linalg.norm(center)
+    slices = [slice(None)] * 3
+    slices[dim] = slice(mid, mid + 1)
+    return omega, dxes, epsilon, e, k_corr, dim, slices
+
+
+def _pml_check(E, shape3, polarity):
+    e = ref.unvec(E, shape3)
+    e_sqr = numpy.squeeze((e.conj() * e).sum(axis=0))
+    tgt, rev = e_sqr[16:19], e_sqr[10:13]
+    if polarity < 0:
+        tgt, rev = rev, tgt
+    numpy.testing.assert_allclose(rev, 0, atol=1e-12)
+    numpy.testing.assert_allclose(tgt, 1, rtol=3e-6)
+
+
+@pytest.mark.parametrize('shape3', [(30, 1, 1), (1, 30, 1), (1, 1, 30)])
+@pytest.mark.parametrize('polarity', [1, -1])
+@pytest.mark.parametrize('eps_fg', [1.0, 2.5])
+@pytest.mark.parametrize('dx', [1.0, 1.5])
+def test_one_way_source_oracle(shape3, polarity, eps_fg, dx):
+    """The reference's test_pml on the oracle: compute_source launches one way only."""
+    omega, dxes, epsilon, e, k, dim, slices = _pml_case(shape3, polarity, eps_fg, dx, None)
+    J = ref.compute_source(e, k, omega, dxes, dim, polarity, slices, epsilon)
+    E = ref.generic(omega, dxes, ref.vec(J), ref.vec(epsilon), matrix_solver_opts=dict(atol=1e-15, rtol=1e-11))
+    _pml_check(E, shape3, polarity)
