@@ -13,10 +13,25 @@ Mirrors the matrix-free forms of the reference:
   (`meanas/fdfd/operators.py:310-336`), the form the reference's `test_poynting_planes`
   uses (`meanas/test/test_fdfd.py:21-55`).
 
+It also gives the wave operators as callables:
+
+- `curl_forward(dx_e)` / `curl_back(dx_h)` (`meanas/fdmath/functional.py`): the forward
+  curl runs the e2h kernel with unit scale; the backward curl is the forward curl of the
+  index-reversed field on the reversed widths, negated (R S R = S^-1 for the periodic
+  shift turns backward differences into negated forward ones);
+- `e_full(omega, dxes, epsilon, mu, pec, pmc)`: the sparse `meanas/fdfd/operators.py:40-96`
+  operator (1/mu divides, as on the solve path; the reference's functional form multiplies
+  by mu instead), applied as (1/p) A (p E) with the solver's symmetrised FP64 apply;
+- `h_full(...)` (`meanas/fdfd/operators.py:126-180`) through the exact duality
+  h_full(omega, dxes, eps, mu, pec, pmc) = R e_full(omega, [rev dxh, rev dxe], rev mu,
+  rev eps, pec = rev pmc, pmc = rev pec) R;
+- `eh_full(...)` (`meanas/fdfd/operators.py:183-250`): `f(E, H) -> (J, -M)` from the two
+  curls and the diagonal terms.
+
 Fields may be given as `(3, nx, ny, nz)` arrays (the functional layout) or as `vec()`
 vectors of length 3N.  numpy inputs give numpy outputs of the same shape; torch tensors
-stay on the device.  Both kernels run through the C ABI (`mx_e2h`,
-`mx_poynting_e_cross_h`).
+stay on the device.  The kernels run through the C ABI (`mx_e2h`,
+`mx_poynting_e_cross_h`, `mx_map`, `mx_apply`).
 """
 from __future__ import annotations
 
@@ -27,7 +42,7 @@ import numpy
 import torch
 
 from .. import _lib
-from .device import torch_stream_handle
+from .device import DeviceOperator, torch_stream_handle
 
 
 class _Grid:
@@ -113,3 +128,145 @@ def poynting(e, h, dxes, device: str | torch.device = 'cuda'):
     """Time-averaged Poynting vector Re(E x H*) / 2 (real, same layout as `e`)."""
     s = poynting_e_cross_h(dxes, device)(e, h, conj_h=True)
     return s.real / 2
+
+
+# ---- curls and wave operators -----------------------------------------------------
+def _flip(t: torch.Tensor, shape) -> torch.Tensor:
+    """R: every cell index reversed, components kept (flat 3N tensor)."""
+    return t.reshape(3, *shape).flip(1, 2, 3).reshape(-1).contiguous()
+
+
+def _curl_f(grid: _Grid, scale: complex) -> Callable[[torch.Tensor], torch.Tensor]:
+    """t -> scale * curl_forward(t) on grid.dxe (device tensors); the e2h kernel with
+    1 / (-i omega) = scale."""
+    omega = 1j / complex(scale)          # 1 / (-i omega) = scale
+
+    def f(t: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(t)
+        _lib.check(_lib.lib().mx_e2h(ctypes.byref(grid.desc), omega.real, omega.imag, None, None,
+                                     t.data_ptr(), out.data_ptr(), torch_stream_handle()), 'mx_e2h')
+        return out
+    return f
+
+
+def _reversed_grid(dx, device) -> _Grid:
+    rev = [numpy.asarray(d)[::-1] for d in dx]
+    return _Grid([rev, rev], device)
+
+
+def curl_forward(dx_e, device: str | torch.device = 'cuda') -> Callable:
+    """`f(E) -> curl_forward(E)` with forward differences on dx_e (fdmath/functional.py)."""
+    grid = _Grid([dx_e, dx_e], device)
+    cf = _curl_f(grid, 1.0)
+
+    def f(e):
+        et, like = _to_device(e, grid, 'E')
+        return _from_device(cf(et), like)
+    return f
+
+
+def curl_back(dx_h, device: str | torch.device = 'cuda') -> Callable:
+    """`f(H) -> curl_back(H)` with backward differences on dx_h (fdmath/functional.py):
+    -R curl_forward(rev dx_h) R."""
+    grid = _reversed_grid(dx_h, device)
+    cf = _curl_f(grid, -1.0)
+
+    def f(h):
+        ht, like = _to_device(h, grid, 'H')
+        return _from_device(_flip(cf(_flip(ht, grid.shape)), grid.shape), like)
+    return f
+
+
+def _vec_or_none(v):
+    return None if v is None else numpy.asarray(v).reshape(-1)
+
+
+def _rev_vec(v, shape):
+    return None if v is None else numpy.ascontiguousarray(numpy.asarray(v).reshape(3, *shape)[:, ::-1, ::-1, ::-1]).reshape(-1)
+
+
+class _A0:
+    """E -> e_full(omega, dxes, epsilon, mu, pec, pmc) @ E on the GPU: (1/p) A (p E)."""
+
+    def __init__(self, omega, dxes, epsilon, mu, pec, pmc, device):
+        self.op = DeviceOperator(omega, dxes, numpy.asarray(epsilon).reshape(-1), mu=_vec_or_none(mu),
+                                 pec=_vec_or_none(pec), pmc=_vec_or_none(pmc), device=device)
+
+    def __call__(self, x: torch.Tensor) -> torch.Tensor:
+        px = self.op.map(_lib.MX_MAP_RHS, x, coef=1.0)
+        return self.op.map(_lib.MX_MAP_E, self.op.apply(px))
+
+
+def _dev_field(f, op: DeviceOperator):
+    if isinstance(f, torch.Tensor):
+        t = f.to(device=op.device, dtype=torch.complex128).reshape(-1).contiguous()
+        like = ('torch', tuple(f.shape))
+    else:
+        a = numpy.asarray(f)
+        t = torch.from_numpy(numpy.ascontiguousarray(a.reshape(-1), dtype=numpy.complex128)).to(op.device)
+        like = ('numpy', a.shape)
+    if t.numel() != op.n3:
+        raise ValueError(f'field has {t.numel()} entries, the grid needs 3N = {op.n3}')
+    return t, like
+
+
+def e_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None,
+           device: str | torch.device = 'cuda') -> Callable:
+    """`f(E) -> e_full @ E` (fdfd/operators.py:40-96)."""
+    A0 = _A0(omega, dxes, epsilon, mu, pec, pmc, device)
+
+    def f(e):
+        et, like = _dev_field(e, A0.op)
+        return _from_device(A0(et), like)
+    return f
+
+
+def h_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None,
+           device: str | torch.device = 'cuda') -> Callable:
+    """`f(H) -> h_full @ H` (fdfd/operators.py:126-180), through the reversed dual e_full."""
+    shape = tuple(len(d) for d in dxes[0])
+    eps = numpy.asarray(epsilon).reshape(-1)
+    dual_dxes = [[numpy.asarray(d)[::-1] for d in dxes[1]], [numpy.asarray(d)[::-1] for d in dxes[0]]]
+    dual_eps = _rev_vec(mu, shape) if mu is not None else numpy.ones_like(eps)
+    A0 = _A0(omega, dual_dxes, dual_eps, _rev_vec(eps, shape), _rev_vec(pmc, shape), _rev_vec(pec, shape), device)
+
+    def f(h):
+        ht, like = _dev_field(h, A0.op)
+        return _from_device(_flip(A0(_flip(ht, shape)), shape), like)
+    return f
+
+
+def eh_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None,
+            device: str | torch.device = 'cuda') -> Callable:
+    """`f(E, H) -> (J, -M)`: [[-pe (i w eps) pe, pe ch pm], [pm ce pe, pm (i w mu) pm]]
+    applied to (E, H) (fdfd/operators.py:183-250)."""
+    ge = _Grid([dxes[0], dxes[0]], device)
+    gh = _reversed_grid(dxes[1], device)
+    ce = _curl_f(ge, 1.0)
+    ch_r = _curl_f(gh, -1.0)
+    shape = ge.shape
+    dev = ge.device
+    w = complex(omega)
+
+    def keep(mask):
+        if mask is None:
+            return None
+        return torch.from_numpy((numpy.asarray(mask).reshape(-1) == 0).astype(numpy.float64)).to(dev)
+
+    pe, pm = keep(pec), keep(pmc)
+    eps = torch.from_numpy(numpy.asarray(epsilon, dtype=numpy.complex128).reshape(-1)).to(dev)
+    mu_t = None if mu is None else torch.from_numpy(numpy.asarray(mu, dtype=numpy.complex128).reshape(-1)).to(dev)
+
+    def m(v, k):
+        return v if k is None else v * k
+
+    def f(e, h):
+        et, like = _to_device(e, ge, 'E')
+        ht, _ = _to_device(h, ge, 'H')
+        pe_e, pm_h = m(et, pe), m(ht, pm)
+        ch_h = _flip(ch_r(_flip(pm_h, shape)), shape)
+        j = m(-1j * w * eps * pe_e + ch_h, pe)
+        iwm_h = 1j * w * (pm_h if mu_t is None else mu_t * pm_h)
+        mm = m(ce(pe_e) + iwm_h, pm)
+        return _from_device(j, like), _from_device(mm, like)
+    return f

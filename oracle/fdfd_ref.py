@@ -15,6 +15,7 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdfd/operators.py:310-336        poynting_e_cross
   meanas/fdfd/scpml.py:18-38,98-167       prepare_s_function / stretch_with_scpml
   meanas/fdfd/solvers.py:21-141           _scipy_qmr / generic
+  meanas/fdfd/operators.py:126-250        h_full / eh_full
   meanas/fdmath/operators.py:55-99        shift_with_mirror
   meanas/fdfd/operators.py:366-450        e_tfsf_source / e_boundary_source
   meanas/fdfd/waveguide_3d.py:113-156,206-252  compute_source / expand_e
@@ -42,6 +43,12 @@ How each function is pinned (tests/test_oracle.py):
                        discrete Green's function through a PML-terminated line.
   e2h / poynting    -- pinned through test_poynting_planes (energy balance), and
                        against roll-based restatements of fdfd/functional.py:89-117,187-221.
+  h_full            -- equals R e_full(omega, [rev dxh, rev dxe], rev mu, rev eps,
+                       pec=rev pmc, pmc=rev pec) R, R the reversal of every cell index
+                       (R S R = S^-1 turns backward differences into negated forward
+                       ones); a swapped role of eps/mu or pec/pmc fails it.
+  eh_full           -- with H = e2h(E) (no M, no masks) its second block row vanishes
+                       and -i omega times the first equals e_full @ E.
   shift_with_mirror -- explicit index table on a small grid (mirror at both edges).
   sources           -- e_tfsf_source, e_boundary_source, expand_e and compute_source
                        pinned together by the restated one-way PML test
@@ -170,6 +177,33 @@ def e_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None) -> sparse
     m_div = sparse.eye_array(n) if mu is None else sparse.diags_array(1 / numpy.asarray(mu))
     op = pe @ (ch @ pm @ m_div @ ce - omega ** 2 * sparse.diags_array(epsilon)) @ pe
     return op.tocsr()
+
+
+def h_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None) -> sparse.csr_array:
+    """pm (ce pe diag(1/eps) ch - omega^2 diag(mu)) pm (fdfd/operators.py:126-180)."""
+    eps = numpy.asarray(epsilon)
+    n = eps.size
+    ch = curl_back(dxes[1])
+    ce = curl_forward(dxes[0])
+    pe = _mask_keep(pec, n)
+    pm = _mask_keep(pmc, n)
+    m = sparse.eye_array(n) if mu is None else sparse.diags_array(numpy.asarray(mu))
+    return (pm @ (ce @ pe @ sparse.diags_array(1 / eps) @ ch - omega ** 2 * m) @ pm).tocsr()
+
+
+def eh_full(omega: complex, dxes, epsilon, mu=None, pec=None, pmc=None) -> sparse.csr_array:
+    """[[-pe (i w eps) pe, pe ch pm], [pm ce pe, pm (i w mu) pm]] acting on [E; H]
+    (fdfd/operators.py:183-250)."""
+    eps = numpy.asarray(epsilon)
+    n = eps.size
+    pe = _mask_keep(pec, n)
+    pm = _mask_keep(pmc, n)
+    iwe = pe @ (1j * omega * sparse.diags_array(eps)) @ pe
+    iwm = 1j * omega * (sparse.eye_array(n) if mu is None else sparse.diags_array(numpy.asarray(mu)))
+    iwm = pm @ iwm @ pm
+    a1 = pe @ curl_back(dxes[1]) @ pm
+    a2 = pm @ curl_forward(dxes[0]) @ pe
+    return sparse.block_array([[-iwe, a1], [a2, iwm]], format='csr')
 
 
 def e_full_preconditioner_vector(dxes) -> numpy.ndarray:

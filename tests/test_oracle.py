@@ -432,3 +432,48 @@ def test_one_way_source_oracle(shape3, polarity, eps_fg, dx):
     J = ref.compute_source(e, k, omega, dxes, dim, polarity, slices, epsilon)
     E = ref.generic(omega, dxes, ref.vec(J), ref.vec(epsilon), matrix_solver_opts=dict(atol=1e-15, rtol=1e-11))
     _pml_check(E, shape3, polarity)
+
+
+def _rev(v, shape):
+    """R: reverse every cell index of a 3N field (components kept)."""
+    return numpy.asarray(v).reshape(3, *shape)[:, ::-1, ::-1, ::-1].reshape(-1)
+
+
+def test_h_full_is_reversed_e_full():
+    """h_full(omega, dxes, eps, mu, pec, pmc) = R e_full(omega, [rev dxh, rev dxe], rev mu,
+    rev eps, pec=rev pmc, pmc=rev pec) R (R S R = S^-1 for the periodic shift)."""
+    rng = numpy.random.default_rng(21)
+    shape = (5, 4, 6)
+    n3 = 3 * 120
+    dxes = [[0.7 + rng.random(k) + 0.2j * rng.random(k) for k in shape] for _ in range(2)]
+    eps = 1 + rng.random(n3) + 0.1j * rng.random(n3)
+    mu = 1 + rng.random(n3)
+    pec = rng.random(n3) < 0.1
+    pmc = rng.random(n3) < 0.1
+    omega = 0.37 - 0.01j
+    H = ref.h_full(omega, dxes, eps, mu, pec, pmc)
+    dual = [[d[::-1] for d in dxes[1]], [d[::-1] for d in dxes[0]]]
+    E = ref.e_full(omega, dual, _rev(mu, shape), _rev(eps, shape), pec=_rev(pmc, shape), pmc=_rev(pec, shape))
+    v = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    numpy.testing.assert_allclose(H @ v, _rev(E @ _rev(v, shape), shape), rtol=1e-12, atol=1e-12)
+    # mu = None is mu = 1 (then the dual problem has eps = 1)
+    numpy.testing.assert_allclose(ref.h_full(omega, dxes, eps) @ v,
+                                  _rev(ref.e_full(omega, dual, numpy.ones(n3), _rev(eps, shape)) @ _rev(v, shape),
+                                       shape), rtol=1e-12, atol=1e-12)
+
+
+def test_eh_full_reduces_to_e_full():
+    """With H = e2h(E) the second block row of eh_full vanishes and -i omega times the
+    first equals e_full @ E (no M, no masks)."""
+    rng = numpy.random.default_rng(22)
+    shape = (4, 5, 3)
+    n3 = 3 * 60
+    dxes = [[0.8 + rng.random(k) for k in shape] for _ in range(2)]
+    eps = 1 + rng.random(n3)
+    mu = 1 + rng.random(n3)
+    omega = 0.45
+    e = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    h = ref.e2h(omega, dxes, mu) @ e
+    out = ref.eh_full(omega, dxes, eps, mu) @ numpy.concatenate([e, h])
+    numpy.testing.assert_allclose(out[n3:], 0, atol=1e-12)
+    numpy.testing.assert_allclose(-1j * omega * out[:n3], ref.e_full(omega, dxes, eps, mu) @ e, rtol=1e-12)
