@@ -162,6 +162,8 @@ enum : int {
   SF_STALL     = 512,  // stopped: replacements no longer reduce the true residual
   SF_SAVE      = 1024, // the last replacement found the best true residual: save x
   SF_FUSED     = 2048, // fused iteration form (complex128 inner vectors; see vector.hip)
+  SF_SWITCH_EXIT = 4096,  // on losing track, finish (SF_SWITCHED) instead of switching form in place
+  SF_SWITCHED  = 8192, // finished to be continued in the classic form by the host
 };
 
 // |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown

@@ -193,6 +193,7 @@ struct Batch {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_chunk = 0;
+  bool classic_launch = true;   // enqueue the classic-form direction update every iteration
 
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -299,7 +300,7 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
-    MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
+    if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
@@ -613,6 +614,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   if (!valid_options(o, &why)) return fail(why);
   Batch B;
   if (build_batch(B, ops, nops, systems, nsys, o)) return -1;
+  B.classic_launch = o.form != MX_FORM_FUSED;
   hipStream_t user = reinterpret_cast<hipStream_t>(stream);
   hipStream_t s = B.stream;
   // order after the caller's pending work (inputs are produced on the caller's stream)
@@ -625,7 +627,9 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   for (int i = 0; i < nsys; ++i) {
     SysState& st = init[i];
     std::memset(&st, 0, sizeof(st));
-    st.flags = SF_PAUSE | SF_RESTART | (o.form == MX_FORM_FUSED ? SF_FUSED : 0);
+    // fused systems that lose track finish and continue below in the classic form, so
+    // the fused iteration needs no direction-update launch
+    st.flags = SF_PAUSE | SF_RESTART | (o.form == MX_FORM_FUSED ? SF_FUSED | SF_SWITCH_EXIT : 0);
     st.rtol = systems[i].rtol;
     st.atol = systems[i].atol;
     st.maxiter = std::max(1, systems[i].maxiter);
@@ -693,8 +697,34 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
       MX_TRY(hipMemcpyAsync(B.h_sys[i].vec[VS_X], B.h_sys[i].vec[VS_XBEST], (size_t)B.h_sys[i].n3 * 16,
                             hipMemcpyDeviceToDevice, s));
   MX_TRY(hipStreamSynchronize(s));
+  std::vector<mx_stats> st_out(nsys);
+  for (int i = 0; i < nsys; ++i) fill_stats(fin[i], &st_out[i]);
+  // systems whose fused recursion lost track continue in the classic form from their
+  // (best) iterate, with the iterations they have left
+  std::vector<int> again;
+  for (int i = 0; i < nsys; ++i)
+    if (fin[i].flags & SF_SWITCHED) again.push_back(i);
+  if (!again.empty()) {
+    std::vector<mx_system> sub;
+    for (int i : again) {
+      mx_system y = systems[i];
+      y.maxiter = std::max(1, systems[i].maxiter - fin[i].iters);
+      sub.push_back(y);
+    }
+    mx_options oc = o;
+    oc.form = MX_FORM_CLASSIC;
+    std::vector<mx_stats> more(sub.size());
+    if (mx_solve(ops, nops, sub.data(), (int32_t)sub.size(), &oc, more.data(), s)) return -1;
+    for (size_t k = 0; k < again.size(); ++k) {
+      mx_stats& t = st_out[again[k]];
+      const int it = t.iterations, rep = t.replacements;
+      t = more[k];
+      t.iterations += it;
+      t.replacements += rep;
+    }
+  }
   if (stats)
-    for (int i = 0; i < nsys; ++i) fill_stats(fin[i], &stats[i]);
+    for (int i = 0; i < nsys; ++i) stats[i] = st_out[i];
   MX_TRY(hipEventRecord(B.ev[0], s));
   MX_TRY(hipStreamWaitEvent(user, B.ev[0], 0));
   return 0;
