@@ -61,10 +61,11 @@ def parse():
                     help='only run the operator apply (complex64 and complex128, default tiling) and the '
                          'kernels of one FP64 solver iteration: the program the hardware-counter profiles '
                          'are taken on')
-    ap.add_argument('--workload', choices=['single', 'multi_rhs', 'uneven', 'serve'], default='single',
+    ap.add_argument('--workload', choices=['single', 'multi_rhs', 'uneven', 'serve', 'fdtd'], default='single',
                     help='single: the 128^3 solve (the metric); multi_rhs: 16 right-hand sides on one '
                          '128x96x96 anisotropic structure; uneven: 32 structures of uneven size with PEC; '
-                         'serve: Poisson request stream with continuous batching')
+                         'serve: Poisson request stream with continuous batching; fdtd: 256^3 Yee leapfrog '
+                         'time stepping (float64 and float32)')
     ap.add_argument('--count', type=int, default=0, help='structures (uneven) or requests (serve)')
     ap.add_argument('--qps', type=float, default=4.0, help='serve: arrival rate')
     ap.add_argument('--form', choices=['auto', 'fused', 'classic'], default='auto',
@@ -142,6 +143,56 @@ def apply_sweep(op, dev):
     print(json.dumps(dict(copy_bytes=2 * nbytes, us=round(us, 2), gbs=round(2 * nbytes / (us * 1e-6) / 1e9, 1))))
 
 
+def fdtd_workload(args):
+    """Yee leapfrog steps on a 256^3 grid (larger than the 256 MB Infinity Cache, so the
+    rates are HBM rates): eps a full field, mu = 1, stretched widths."""
+    import torch
+    from meanas_amd import fdtd
+    n = args.size if args.size != 128 else 256
+    shape = (n, n, n)
+    rng = numpy.random.default_rng(9)
+    dxes = [[0.9 + 0.2 * rng.random(k) for k in shape] for _ in range(2)]
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, dt_ in (('f64', torch.float64), ('f32', torch.float32)):
+        elem = 8 if dt_ == torch.float64 else 4
+        e = torch.zeros((3, *shape), dtype=dt_, device='cuda')
+        h = torch.zeros_like(e)
+        eps = (1.0 + torch.rand((3, *shape), dtype=dt_, device='cuda'))
+        e[1, n // 2, n // 2, n // 2] = 1.0
+        me, mh = fdtd.maxwell_e(0.3, dxes), fdtd.maxwell_h(0.3, dxes)
+        for _ in range(3):
+            mh(e, h)
+            me(e, h, eps)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        reps = 20
+        ev[0].record(stream)
+        for _ in range(reps):
+            mh(e, h)
+        ev[1].record(stream)
+        for _ in range(reps):
+            me(e, h, eps)
+        ev[2].record(stream)
+        fdtd.run(e, h, eps, 0.3, dxes, steps=args.steps * 10)
+        ev[3].record(stream)
+        torch.cuda.synchronize()
+        us_h = ev[0].elapsed_time(ev[1]) * 1e3 / reps
+        us_e = ev[1].elapsed_time(ev[2]) * 1e3 / reps
+        us_step = ev[2].elapsed_time(ev[3]) * 1e3 / (args.steps * 10)
+        cells = n ** 3
+        b_h = 3 * elem * 3 * cells                 # e read, h read, h written
+        b_e = 4 * 3 * elem * cells                 # h, e, eps read, e written
+        out[name] = dict(step_us=round(us_step, 2), mcell_updates_per_s=round(cells / us_step, 1),
+                         h_update=dict(us=round(us_h, 2), bytes=b_h, gbs=round(b_h / us_h / 1e3, 1)),
+                         e_update=dict(us=round(us_e, 2), bytes=b_e, gbs=round(b_e / us_e / 1e3, 1)))
+        log(f'fdtd {name}: {out[name]}')
+        del e, h, eps
+    line = dict(metric='FDTD cell updates/s (Yee leapfrog step: H then E update)', value=out['f64']['mcell_updates_per_s'] * 1e6,
+                unit='cell-updates/s', n_gpus=1, dtype='f64 (f32 beside it)', data='synthetic',
+                config={'workload': f'fdtd: {n}^3 grid, eps field, mu = 1, stretched widths'}, **out)
+    print(json.dumps(line), flush=True)
+
+
 def batch_workloads(args):
     """Multi-RHS, uneven multi-structure and serving workloads (one JSON line each)."""
     import torch
@@ -150,6 +201,9 @@ def batch_workloads(args):
     dev = torch.device('cuda:0')
     mode = _lib.MX_FP64 if args.mode == 'fp64' else _lib.MX_MIXED
     fopt = dict(form=args.form if (mode == _lib.MX_FP64 or args.form != 'fused') else 'auto')
+    if args.workload == 'fdtd':
+        fdtd_workload(args)
+        return
     if args.workload == 'multi_rhs':
         w = workloads.aniso_multi()
         op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)

@@ -189,6 +189,31 @@ int mx_e2h(const mx_grid* g, double omega_re, double omega_im, const void* inv_m
  * that conjugation in the kernel.  e, h, s: 3N complex128. */
 int mx_poynting_e_cross_h(const mx_grid* g, const void* e, const void* h, int32_t conj_h, void* s, void* stream);
 
+/* ---- time domain (FDTD) ------------------------------------------------------------
+ * Real fields (3, nx, ny, nz) in vec() order, float32 (MX_REAL32) or float64 (MX_REAL64).
+ * Grid: real cell widths dxes[0] (E grid) and dxes[1] (H grid), float64 device arrays of
+ * lengths nx, ny, nz; NULL means unit widths (the reference's dxes=None). */
+#define MX_REAL32 0
+#define MX_REAL64 1
+typedef struct mx_fdtd_grid {
+  int32_t nx, ny, nz, pad;
+  const double* dxe[3];
+  const double* dxh[3];
+} mx_fdtd_grid;
+
+/* h -= dt curl_forward(e) / mu (fdtd/base.py:67-125, maxwell_h).  mu: 3N values in the
+ * field precision, or NULL to divide by mu_scalar. */
+int mx_fdtd_h(const mx_fdtd_grid* g, int32_t prec, double dt, const void* e, void* h, const void* mu,
+              double mu_scalar, void* stream);
+/* e += dt curl_back(h) / epsilon (fdtd/base.py:13-64, maxwell_e).  eps: 3N values or NULL
+ * to divide by eps_scalar. */
+int mx_fdtd_e(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, const void* h, const void* eps,
+              double eps_scalar, void* stream);
+/* `steps` leapfrog steps: maxwell_h then maxwell_e, as the reference's update loop
+ * (meanas/test/test_fdtd.py:215-233) runs them, without sources. */
+int mx_fdtd_run(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, void* h, const void* eps,
+                double eps_scalar, const void* mu, double mu_scalar, int32_t steps, void* stream);
+
 /* Kernel timing of solver iterations (measurement; no counterpart in the reference).
  * Sets up `sys` as mx_solve would (r = b - A x, p = r) with every stopping rule off, runs
  * `reps` iterations on `stream` exactly as a solve launches them, with a HIP event

@@ -26,6 +26,7 @@ MX_BREAKDOWN = 2
 MX_NONFINITE = 3
 MX_STALLED = 4
 MX_FORM_AUTO, MX_FORM_FUSED, MX_FORM_CLASSIC = 0, 1, 2
+MX_REAL32, MX_REAL64 = 0, 1
 FORMS = {'auto': MX_FORM_AUTO, 'fused': MX_FORM_FUSED, 'classic': MX_FORM_CLASSIC}
 STATUS_NAMES = {0: 'converged', 1: 'maxiter', 2: 'breakdown', 3: 'nonfinite', 4: 'stalled'}
 
@@ -75,6 +76,12 @@ class MxGrid(ctypes.Structure):
                 ('dxe', c_void_p * 3), ('dxh', c_void_p * 3)]
 
 
+class MxFdtdGrid(ctypes.Structure):
+    """mx_fdtd_grid: real cell widths (float64 device arrays) or NULL for unit widths."""
+    _fields_ = [('nx', ctypes.c_int32), ('ny', ctypes.c_int32), ('nz', ctypes.c_int32), ('pad', ctypes.c_int32),
+                ('dxe', c_void_p * 3), ('dxh', c_void_p * 3)]
+
+
 class MxStats(ctypes.Structure):
     _fields_ = [
         ('status', ctypes.c_int32), ('iterations', ctypes.c_int32),
@@ -108,6 +115,12 @@ PROTOTYPES = {
                               c_void_p, c_void_p, c_void_p]),
     'mx_poynting_e_cross_h': (ctypes.c_int, [ctypes.POINTER(MxGrid), c_void_p, c_void_p, ctypes.c_int32, c_void_p,
                                              c_void_p]),
+    'mx_fdtd_h': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_double, c_void_p, c_void_p,
+                                 c_void_p, ctypes.c_double, c_void_p]),
+    'mx_fdtd_e': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_double, c_void_p, c_void_p,
+                                 c_void_p, ctypes.c_double, c_void_p]),
+    'mx_fdtd_run': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_double, c_void_p, c_void_p,
+                                   c_void_p, ctypes.c_double, c_void_p, ctypes.c_double, ctypes.c_int32, c_void_p]),
     'mx_time_kernels': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), ctypes.POINTER(MxOptions),
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, c_void_p]),
     'mx_session_create': (c_void_p, [ctypes.c_int32, ctypes.POINTER(MxOptions)]),

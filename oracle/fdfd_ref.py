@@ -17,6 +17,10 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdfd/solvers.py:21-141           _scipy_qmr / generic
   meanas/fdfd/operators.py:126-250        h_full / eh_full
   meanas/fdmath/operators.py:55-99        shift_with_mirror
+  meanas/fdmath/functional.py             curl_forward / curl_back (callables, dx None = 1)
+  meanas/fdtd/base.py:13-125              maxwell_e / maxwell_h
+  meanas/fdtd/energy.py:10-295            poynting, poynting_divergence, energy_hstep,
+                                          energy_estep, delta_energy_j, dxmul
   meanas/fdfd/operators.py:366-450        e_tfsf_source / e_boundary_source
   meanas/fdfd/waveguide_3d.py:113-156,206-252  compute_source / expand_e
 Third-party arithmetic: scipy.sparse (CSR products) and scipy.sparse.linalg.qmr
@@ -50,6 +54,11 @@ How each function is pinned (tests/test_oracle.py):
   eh_full           -- with H = e2h(E) (no M, no masks) its second block row vanishes
                        and -i omega times the first equals e_full @ E.
   shift_with_mirror -- explicit index table on a small grid (mirror at both edges).
+  fdtd              -- the functional curls equal the sparse curl_forward/curl_back; the
+                       updates and the energy functions are pinned together by the
+                       restated reference tests of meanas/test/test_fdtd.py (initial
+                       fields and energy, energy conservation, Poynting divergence and
+                       Poynting planes) over the reference's fixture grid.
   sources           -- e_tfsf_source, e_boundary_source, expand_e and compute_source
                        pinned together by the restated one-way PML test
                        (meanas/test/test_fdfd_pml.py:14-28): the source launches a wave
@@ -442,3 +451,105 @@ def dense_solve(omega: complex, dxes, J, epsilon, mu=None, pec=None, pmc=None,
         z = numpy.linalg.lstsq(A, Pr.conj().T @ b0, rcond=None)[0]
         return Pl.conj().T @ z
     return numpy.linalg.lstsq(A0, b0, rcond=None)[0]
+
+
+# ---------------------------------------------------------------------------
+# FDTD  (meanas/fdmath/functional.py curls; meanas/fdtd/base.py, energy.py)
+# ---------------------------------------------------------------------------
+def _unit_dx(dx):
+    return [numpy.ones(1)] * 3 if dx is None else list(dx)
+
+
+def f_curl_forward(dx_e=None) -> Callable:
+    """f -> curl with forward differences (f[n+1] - f[n]) / dx_e[n] (fdmath/functional.py)."""
+    dx = _unit_dx(dx_e)
+
+    def d(f, a):
+        w = dx[a].reshape([-1 if b == a else 1 for b in range(3)])
+        return (numpy.roll(f, -1, axis=a) - f) / w
+
+    def curl(e):
+        return numpy.stack([d(e[2], 1) - d(e[1], 2), d(e[0], 2) - d(e[2], 0), d(e[1], 0) - d(e[0], 1)])
+    return curl
+
+
+def f_curl_back(dx_h=None) -> Callable:
+    """h -> curl with backward differences (f[n] - f[n-1]) / dx_h[n] (fdmath/functional.py)."""
+    dx = _unit_dx(dx_h)
+
+    def d(f, a):
+        w = dx[a].reshape([-1 if b == a else 1 for b in range(3)])
+        return (f - numpy.roll(f, 1, axis=a)) / w
+
+    def curl(h):
+        return numpy.stack([d(h[2], 1) - d(h[1], 2), d(h[0], 2) - d(h[2], 0), d(h[1], 0) - d(h[0], 1)])
+    return curl
+
+
+def maxwell_e(dt: float, dxes=None) -> Callable:
+    """e += dt curl_back(h) / epsilon, in place (fdtd/base.py:13-64)."""
+    ch = f_curl_back(None if dxes is None else dxes[1])
+
+    def me(e, h, epsilon):
+        e += dt * ch(h) / epsilon
+        return e
+    return me
+
+
+def maxwell_h(dt: float, dxes=None) -> Callable:
+    """h -= dt curl_forward(e) / mu, in place (fdtd/base.py:67-125)."""
+    ce = f_curl_forward(None if dxes is None else dxes[0])
+
+    def mh(e, h, mu=None):
+        h -= dt * ce(e) / (1 if mu is None else mu)
+        return h
+    return mh
+
+
+def _grid_weights(dx):
+    return dx[0][:, None, None] * dx[1][None, :, None] * dx[2][None, None, :]
+
+
+def fdtd_poynting(e, h, dxes=None):
+    """E x H with E shifted forward, width-weighted (fdtd/energy.py:10-75)."""
+    if dxes is None:
+        dxes = [[numpy.ones(1)] * 3] * 2
+    ex, ey, ez = (e[c] * dxes[0][c].reshape([-1 if b == c else 1 for b in range(3)]) for c in range(3))
+    hx, hy, hz = (h[c] * dxes[1][c].reshape([-1 if b == c else 1 for b in range(3)]) for c in range(3))
+    return numpy.stack([numpy.roll(ey, -1, 0) * hz - numpy.roll(ez, -1, 0) * hy,
+                        numpy.roll(ez, -1, 1) * hx - numpy.roll(ex, -1, 1) * hz,
+                        numpy.roll(ex, -1, 2) * hy - numpy.roll(ey, -1, 2) * hx])
+
+
+def poynting_divergence(s=None, *, e=None, h=None, dxes=None):
+    """Backward-difference divergence of S with unit spacing, as the reference computes it
+    (deriv_back() without widths, fdtd/energy.py:78-110)."""
+    if s is None:
+        s = fdtd_poynting(e, h, dxes)
+    return sum(s[a] - numpy.roll(s[a], 1, axis=a) for a in range(3))
+
+
+def dxmul(ee, hh, epsilon=None, mu=None, dxes=None):
+    """sum_c ee eps dV_e + sum_c hh mu dV_h (fdtd/energy.py:268-295)."""
+    epsilon = 1 if epsilon is None else epsilon
+    mu = 1 if mu is None else mu
+    if dxes is None:
+        dxes = [[numpy.ones(1)] * 3] * 2
+    return (ee * epsilon).sum(axis=0) * _grid_weights(dxes[0]) + (hh * mu).sum(axis=0) * _grid_weights(dxes[1])
+
+
+def energy_hstep(e0, h1, e2, epsilon=None, mu=None, dxes=None):
+    """Energy at an H step: dxmul(e0 e2, h1 h1) (fdtd/energy.py:113-140)."""
+    return dxmul(e0 * e2, h1 * h1, epsilon, mu, dxes)
+
+
+def energy_estep(h0, e1, h2, epsilon=None, mu=None, dxes=None):
+    """Energy at an E step: dxmul(e1 e1, h0 h2) (fdtd/energy.py:143-170)."""
+    return dxmul(e1 * e1, h0 * h2, epsilon, mu, dxes)
+
+
+def delta_energy_j(j0, e1, dxes=None):
+    """(j0 . e1) dV_e (fdtd/energy.py:239-265)."""
+    if dxes is None:
+        dxes = [[numpy.ones(1)] * 3] * 2
+    return (j0 * e1).sum(axis=0) * _grid_weights(dxes[0])

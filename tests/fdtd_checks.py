@@ -1,0 +1,157 @@
+"""The reference's FDTD tests (meanas/test/test_fdtd.py) restated over its fixture grid
+(meanas/test/conftest.py), with the time stepping passed in, so the same checks run on
+the oracle's updates and on the GPU's.  The energy functions are the oracle's."""
+from dataclasses import dataclass, field
+
+import numpy
+
+from oracle import fdfd_ref as ref
+
+PRNG = numpy.random.RandomState(12345)
+
+
+@dataclass
+class TDResult:
+    shape: tuple
+    dt: float
+    dxes: list
+    epsilon: numpy.ndarray
+    es: list = field(default_factory=list)
+    hs: list = field(default_factory=list)
+    js: list = field(default_factory=list)
+
+
+def cases():
+    """(shape, eps_kind, bg, fg, dx, dxes_kind, j_kind) as the reference's fixtures combine
+    them, with its 3D skips."""
+    out = []
+    for shape in [(5, 5, 1), (5, 1, 5), (5, 5, 5)]:
+        is3d = 1 not in shape
+        for eps_kind in ['center', '000', 'random']:
+            for bg in [1.0, 1.5]:
+                for fg in [1.0, 2.5]:
+                    if is3d and (eps_kind == '000' or bg != 1 or fg not in (1.0, 2.0)):
+                        continue
+                    for dx in [1.0, 1.5]:
+                        for dxk in ['uniform', 'centerbig']:
+                            for jk in ['center', 'random']:
+                                out.append((shape, eps_kind, bg, fg, dx, dxk, jk))
+    return out
+
+
+def make_sim(case, step, dt=0.3, seed=12345):
+    """10 steps from zero fields with J added at steps 0, 4 and 8 (test_fdtd.py:183-235)."""
+    shape3, eps_kind, bg, fg, dx, dxk, jk = case
+    rng = numpy.random.RandomState(seed)
+    shape = (3, *shape3)
+    eps = numpy.full(shape, bg, dtype=float)
+    if eps_kind == 'center':
+        eps[:, shape3[0] // 2, shape3[1] // 2, shape3[2] // 2] = fg
+    elif eps_kind == '000':
+        eps[:, 0, 0, 0] = fg
+    else:
+        eps[:] = rng.uniform(low=min(bg, fg), high=max(bg, fg), size=shape)
+    dxes = [[numpy.full(s, dx) for s in shape3] for _ in range(2)]
+    if dxk == 'centerbig':
+        for eh in (0, 1):
+            for ax in (0, 1, 2):
+                dxes[eh][ax][dxes[eh][ax].size // 2] *= 1.1
+    j = numpy.zeros(shape)
+    if jk == 'center':
+        j[:, shape3[0] // 2, shape3[1] // 2, shape3[2] // 2] = 1.0
+    else:
+        j[:] = rng.uniform(low=-1.0, high=1.0, size=shape)
+    sim = TDResult(shape=shape, dt=dt, dxes=dxes, epsilon=eps)
+    e = numpy.zeros(shape)
+    h = numpy.zeros(shape)
+    jz = numpy.zeros(shape)
+    for tt in range(10):
+        e = e.copy()
+        h = h.copy()
+        e, h = step(e, h, eps, dt, dxes)
+        if tt in (0, 4, 8):
+            e += j / eps
+            sim.js.append(j)
+        else:
+            sim.js.append(jz)
+        sim.es.append(e)
+        sim.hs.append(h)
+    return sim
+
+
+def oracle_step(e, h, eps, dt, dxes):
+    ref.maxwell_h(dt, dxes)(e, h)
+    ref.maxwell_e(dt, dxes)(e, h, eps)
+    return e, h
+
+
+def close(x, y):
+    numpy.testing.assert_allclose(x, y)
+
+
+def check_initial_fields(sim):
+    e0, h0, j0 = sim.es[0], sim.hs[0], sim.js[0]
+    mask = j0 != 0
+    close(e0[mask], j0[mask] / sim.epsilon[mask])
+    assert not e0[~mask].any()
+    assert not h0.any()
+
+
+def check_initial_energy(sim):
+    j0, e0, h0, h1 = sim.js[0], sim.es[0], sim.hs[0], sim.hs[1]
+    dV = numpy.prod(numpy.meshgrid(*sim.dxes[0], indexing='ij'), axis=0)
+    u0 = (j0 * j0.conj() / sim.epsilon * dV).sum(axis=0)
+    close(ref.energy_estep(h0=h0, e1=e0, h2=h1, dxes=sim.dxes, epsilon=sim.epsilon), u0)
+    close(ref.delta_energy_j(j0=j0, e1=e0, dxes=sim.dxes), u0)
+
+
+def check_energy_conservation(sim):
+    u = ref.delta_energy_j(j0=sim.js[0], e1=sim.es[0], dxes=sim.dxes).sum()
+    for ii in range(1, 8):
+        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        u += ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes).sum()
+        close(u_h.sum(), u)
+        u += ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes).sum()
+        close(u_e.sum(), u)
+
+
+def check_poynting_divergence(sim):
+    u_prev = None
+    for ii in range(1, 8):
+        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        dj_b = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
+        close(u_e - u_h - dj_b, -sim.dt * ref.poynting_divergence(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes))
+        if u_prev is not None:
+            dj_a = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
+            close(u_h - u_prev - dj_a,
+                  -sim.dt * ref.poynting_divergence(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes))
+        u_prev = u_e
+
+
+def check_poynting_planes(sim):
+    mask = (sim.js[0] != 0).any(axis=0)
+    if mask.sum() > 1:
+        return            # the reference skips distributed sources here
+    mx, my, mz = (numpy.roll(mask, -1, axis=a) for a in range(3))
+    u_prev = None
+    for ii in range(1, 8):
+        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        dj_b = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
+        s = -ref.fdtd_poynting(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
+        planes = [s[0, mask].sum(), -s[0, mx].sum(), s[1, mask].sum(), -s[1, my].sum(),
+                  s[2, mask].sum(), -s[2, mz].sum()]
+        close(sum(planes), (u_e - u_h - dj_b)[mask])
+        if u_prev is not None:
+            dj_a = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
+            s = -ref.fdtd_poynting(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
+            planes = [s[0, mask].sum(), -s[0, mx].sum(), s[1, mask].sum(), -s[1, my].sum(),
+                      s[2, mask].sum(), -s[2, mz].sum()]
+            close(sum(planes), (u_h - u_prev - dj_a)[mask])
+        u_prev = u_e
+
+
+CHECKS = [check_initial_fields, check_initial_energy, check_energy_conservation,
+          check_poynting_divergence, check_poynting_planes]
