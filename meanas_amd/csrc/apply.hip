@@ -185,6 +185,10 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   const cplx<T> fa_ym = coef_u(cy, wrapi(j - 1, ny)).fa, fa_zm = coef_at(cz, wrapi(k - 1, nz)).fa;
   const FaFb<T> cyh = coef_u(cy, wrapi(j0 - 1, ny));   // halo row j0-1
   const FaFb<T> czh = coef_u(cz, wrapi(k0 - 1, nz));   // halo column k0-1
+  // y coefficients of the row whose column-halo H this lane computes (lanes tz < TY of
+  // wave TY-1).  Loaded once here: a global load consumed inside the plane loop would
+  // make the wave wait for every plane load in flight (the memory counter is in order).
+  const FaFb<T> cyr = coef_at(cy, wrapi(j0 + (tz < TY ? tz : 0), ny));
 
   // H multiplier at (cell, component): pmc keep * 1/mu
   auto hscale = [&](cplx<T> h, int c, uint8_t m, int cell) -> cplx<T> {
@@ -196,6 +200,13 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     return h;
   };
   auto e_ = [&](int c, int rr, int qq) { return sE[c][rr][qq]; };
+
+  cplx<T> epsn[3];
+  auto load_eps = [&](int cell) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      epsn[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
+  };
 
   // ---- prologue: H(ib-1) at the own column (y, z components); plane ib into LDS
   cplx<T> hpy, hpz;
@@ -216,6 +227,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     hpz = hscale(hz, 2, m, cell);
     __syncthreads();
     commit(R);                      // plane ib
+    load_eps(ib * (int)plane + own_off);
     fetch(RA, wrapi(ib + 1, nx));   // plane ib+1 (H(ie-1) needs plane ie, and ib < ie)
     if constexpr (PD == 2)
       if (ib + 2 <= ie) fetch(RB, wrapi(ib + 2, nx));
@@ -229,13 +241,13 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
   RT* __restrict__ pdir = reinterpret_cast<RT*>(S.vec[VS_P]);
 
-  // step i: Rn holds plane i+1
+  // step i: Rn holds plane i+1; epsn holds eps of plane i, loaded during step i-1 ahead
+  // of that step's plane prefetch (so waiting for it never waits for the prefetch)
   auto step = [&](const int i, Regs& Rn) {
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      epsv[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
+    for (int c = 0; c < 3; ++c) epsv[c] = epsn[c];
     const FaFb<T> cxi = coef_u(cx, i);
     const cplx<T> fa_xm = coef_u(cx, wrapi(i - 1, nx)).fa;
     __syncthreads();   // plane i is in LDS; the previous step's reads of sH are done
@@ -267,7 +279,6 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
       if (col_halo_h) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
         const int rr = tz + 1;
         const int jj = wrapi(j0 + tz, ny);
-        const FaFb<T> cyr = coef_at(cy, jj);
         const uint8_t mh = sM[rr][0];
         const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
@@ -286,6 +297,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     __syncthreads();   // H(i) is in LDS; every read of plane i is done
     if (i + 1 < ie) {
       commit(Rn);                                         // plane i+1 replaces plane i
+      load_eps((i + 1) * (int)plane + own_off);           // eps of plane i+1, ahead of the prefetch
       if (i + 1 + PD <= ie) fetch(Rn, wrapi(i + 1 + PD, nx));   // plane i+1+PD into the freed set
     }
 
