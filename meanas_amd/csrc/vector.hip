@@ -388,9 +388,6 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
   }
   const bool fused = (f & SF_FUSED) != 0;
   const int nblk = (KIND == FIN_APPLY || KIND == FIN_RESID) ? S.nblk_apply : S.nblk_vec;
-  // thread 0 reads the whole scalar state now, so the loads overlap the reduction below
-  SysState v;
-  if (threadIdx.x == 0) v = *st;
   double tot[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (KIND == FIN_BNORM) {
     double t1[1];
@@ -411,19 +408,19 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
       // new residual r - alpha A g generic (r^T r no longer ~ 0); beta = 0 restarts
       // COCG from that residual.
       const double qq = tot[2];
-      const cplx<double> alpha{sqrt(v.rnorm2 / qq), 0.0};
+      const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
       if (!(qq > 0.0) || !finite(alpha.re)) f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0
       else {
-        v.alpha = alpha;
+        st->alpha = alpha;
         if (fused) {   // beta = 0 restarts the fused recurrence (classic: k_pupdate does)
-          v.beta = cplx<double>{0.0, 0.0};
+          st->beta = cplx<double>{0.0, 0.0};
           f &= ~SF_RESTART;
         }
       }
     } else {
       const cplx<double> mu{tot[0], tot[1]}, qtq{tot[2], tot[3]};
-      v.mu = mu;
-      const cplx<double> rho = v.rho;
+      st->mu = mu;
+      const cplx<double> rho = st->rho;
       const cplx<double> alpha = cdiv(rho, mu);
       if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
         // p^T A p vanished (or overflowed): replace the residual, then rescue with a
@@ -434,9 +431,9 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
         if (fused) {   // predicted rho' = alpha^2 q^T q - rho; beta = rho' / rho
           cplx<double> beta = cdiv(alpha * alpha * qtq, rho) - cplx<double>{1.0, 0.0};
           if (!finite(beta.re) || !finite(beta.im)) beta = cplx<double>{0.0, 0.0};
-          v.beta = beta;
+          st->beta = beta;
         }
-        v.alpha = alpha;
+        st->alpha = alpha;
         f &= ~SF_RESTART;
       }
     }
@@ -447,11 +444,11 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     const bool rescued = (f & SF_MR) != 0;
     f &= ~SF_MR;       // a rescue step (if that was one) is done
     if (!fused) {      // classic: beta from the exact rho'; a rescue step restarts
-      if (rescued) v.beta = cplx<double>{0.0, 0.0};
-      else if (norm2(v.rho) > 0.0) v.beta = cdiv(rho, v.rho);
+      if (rescued) st->beta = cplx<double>{0.0, 0.0};
+      else if (norm2(st->rho) > 0.0) st->beta = cdiv(rho, st->rho);
       else {           // rho breakdown: restart from the replaced residual
         f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
-        v.beta = cplx<double>{0.0, 0.0};
+        st->beta = cplx<double>{0.0, 0.0};
       }
     }
     // r^T r ~ 0 with r != 0: the next step is a rescue step along a hashed direction.
@@ -459,74 +456,73 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     // which k_pupdate writes the rescue direction.
     if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0)
       f |= SF_MR | SF_RESTART | (fused ? SF_PAUSE : 0);
-    v.rho_old = v.rho;
-    v.rho = rho;
-    v.rnorm2 = rn2;
-    if (rn2 > v.rmax2) v.rmax2 = rn2;
-    v.iters += 1;
-    v.since += 1;
+    st->rho_old = st->rho;
+    st->rho = rho;
+    st->rnorm2 = rn2;
+    if (rn2 > st->rmax2) st->rmax2 = rn2;
+    st->iters += 1;
+    st->since += 1;
     if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
       f |= SF_NAN | SF_PAUSE | SF_LIMIT;
-    } else if (rn2 <= v.stop2 || rn2 <= v.delta2 * v.rmax2 ||
-               (v.refresh_iters > 0 && v.since >= v.refresh_iters)) {
+    } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
+               (st->refresh_iters > 0 && st->since >= st->refresh_iters)) {
       f |= SF_PAUSE;
     }
-    if (v.iters >= v.maxiter) f |= SF_PAUSE | SF_LIMIT;
+    if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
   } else if constexpr (KIND == FIN_RESID) {
     // residual replacement / initial residual
     const double rn2 = tot[0];
     const cplx<double> rho{tot[1], tot[2]};
-    v.true_rnorm2 = rn2;
-    v.rnorm_rec2 = v.rnorm2;   // recursive residual the replacement overwrites
-    v.rnorm2 = rn2;
-    v.rmax2 = rn2;
-    v.replacements += 1;
-    v.since = 0;
+    st->true_rnorm2 = rn2;
+    st->rnorm_rec2 = st->rnorm2;   // recursive residual the replacement overwrites
+    st->rnorm2 = rn2;
+    st->rmax2 = rn2;
+    st->replacements += 1;
+    st->since = 0;
     // stagnation: (a) the recursive residual had drifted >= 100x below the true one at
     // stall_limit consecutive replacements (the inner precision no longer tracks the
     // operator), or (b) the best true residual has not halved within stall_iters
     // iterations.  COCG's residual oscillates, so single replacements prove nothing.
-    const bool first = v.replacements == 1;
-    if (!first && v.rnorm_rec2 < 1e-4 * rn2) v.stalls += 1;
-    else v.stalls = 0;
-    if (first || rn2 < 0.25 * v.mark2) {
-      v.mark2 = rn2;
-      v.mark_iter = v.iters;
+    const bool first = st->replacements == 1;
+    if (!first && st->rnorm_rec2 < 1e-4 * rn2) st->stalls += 1;
+    else st->stalls = 0;
+    if (first || rn2 < 0.25 * st->mark2) {
+      st->mark2 = rn2;
+      st->mark_iter = st->iters;
     }
-    if (first || rn2 < v.best_true2) {
-      v.best_true2 = rn2;
+    if (first || rn2 < st->best_true2) {
+      st->best_true2 = rn2;
       f |= SF_SAVE;
     }
     // a fused recursion that lost track of the true residual continues in the classic
     // form, from a restart (p was built with the predicted beta)
-    if (fused && !first && v.rnorm_rec2 < 1e-2 * rn2) {
+    if (fused && !first && st->rnorm_rec2 < 1e-2 * rn2) {
       if (f & SF_SWITCH_EXIT) f |= SF_SWITCHED | SF_DONE;   // the host continues it in the classic form
       else f = (f & ~SF_FUSED) | SF_RESTART;
     }
-    if (v.stall_limit > 0 && v.stalls >= v.stall_limit) f |= SF_STALL | SF_DONE;
-    if (v.stall_iters > 0 && v.iters - v.mark_iter > v.stall_iters) f |= SF_STALL | SF_DONE;
+    if (st->stall_limit > 0 && st->stalls >= st->stall_limit) f |= SF_STALL | SF_DONE;
+    if (st->stall_iters > 0 && st->iters - st->mark_iter > st->stall_iters) f |= SF_STALL | SF_DONE;
     if (!finite(rn2)) f |= SF_NAN | SF_DONE;
-    if (rn2 <= v.stop2 || (f & (SF_DONE | SF_LIMIT))) {
+    if (rn2 <= st->stop2 || (f & (SF_DONE | SF_LIMIT))) {
       f |= SF_DONE;
     } else {
       // fused: the replacement kernel already moved p onto the true residual (unless a
       // restart or a rescue step is pending: k_pupdate rebuilds p then).  Classic: the
       // next k_pupdate continues the direction with beta = rho_true / rho_old.
       if (!(f & SF_FUSED)) {
-        const cplx<double> ro = v.rho_old;
-        v.beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
+        const cplx<double> ro = st->rho_old;
+        st->beta = (norm2(ro) > 0.0) ? cdiv(rho, ro) : cplx<double>{0.0, 0.0};
       }
-      v.rho = rho;
+      st->rho = rho;
       if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2) f |= SF_RESTART | SF_MR;
       f &= ~(SF_PAUSE | SF_BREAKDOWN);
     }
   } else {   // FIN_BNORM
-    v.bnorm2 = tot[0];
-    const double a = v.rtol * v.rtol * tot[0], c = v.atol * v.atol;
-    v.stop2 = a > c ? a : c;
+    st->bnorm2 = tot[0];
+    const double a = st->rtol * st->rtol * tot[0], c = st->atol * st->atol;
+    st->stop2 = a > c ? a : c;
   }
-  v.flags = f;
-  *st = v;
+  st->flags = f;
 }
 
 // ---- host launch helpers ----------------------------------------------------------
