@@ -164,6 +164,7 @@ enum : int {
   SF_FUSED     = 2048, // fused iteration form (complex128 inner vectors; see vector.hip)
   SF_SWITCH_EXIT = 4096,  // on losing track, finish (SF_SWITCHED) instead of switching form in place
   SF_SWITCHED  = 8192, // finished to be continued in the classic form by the host
+  SF_UPD_PENDING = 16384, // the update ran; its partials await the post-update finishing step
 };
 
 // |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown
@@ -201,6 +202,9 @@ struct SysState {
   int mark_iter;           // iteration of the last 2x improvement
 };
 
+// Finishing-kernel kinds (k_fin<KIND>, vector.hip)
+enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3, FIN_APPLY_MERGED = 4 };
+
 // One linear system of a batch.  Vector slots name the vectors a kernel reads/writes.
 enum : int { VS_X = 0, VS_R, VS_P, VS_Q, VS_XLO, VS_B, VS_XBEST, VS_COUNT };
 
@@ -213,6 +217,8 @@ struct SysDesc {
   void* vec[VS_COUNT];        // device vectors (precision depends on the slot and mode)
   SysState* st;
   double* partial;            // >= 4 * max(nblk_apply, nblk_vec) doubles
+  double* partial_vec;        // >= 3 * nblk_vec doubles: the update's partials, kept apart
+                              // so the next apply can run before they are summed
 };
 
 // Work assignment of one block: system index and block index within that system.
@@ -285,6 +291,37 @@ __device__ __forceinline__ void final_sum(double (&out)[K], const double* partia
     out[q] = sacc;
   }
   block_sum<K>(out, scratch);
+}
+
+// Fixed-order sums of two partial sets in one pass: every load of both is issued before
+// any is summed, so the two reductions cost one memory round trip.
+template <int K1, int K2>
+__device__ __forceinline__ void final_sum2(double (&o1)[K1], const double* p1, int n1, double (&o2)[K2],
+                                           const double* p2, int n2, double* scratch) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthreads = blockDim.x * blockDim.y;
+  double v1[K1][kFinLoads], v2[K2][kFinLoads];
+#pragma unroll
+  for (int l = 0; l < kFinLoads; ++l) {
+    const int b = tid + l * nthreads;
+#pragma unroll
+    for (int q = 0; q < K1; ++q) v1[q][l] = b < n1 ? gload1<double>(p1, q * n1 + b) : 0.0;
+#pragma unroll
+    for (int q = 0; q < K2; ++q) v2[q][l] = b < n2 ? gload1<double>(p2, q * n2 + b) : 0.0;
+  }
+  double all[K1 + K2];
+#pragma unroll
+  for (int q = 0; q < K1 + K2; ++q) {
+    double sacc = 0.0;
+#pragma unroll
+    for (int l = 0; l < kFinLoads; ++l) sacc += q < K1 ? v1[q][l] : v2[q - K1][l];
+    all[q] = sacc;
+  }
+  block_sum<K1 + K2>(all, scratch);
+#pragma unroll
+  for (int q = 0; q < K1; ++q) o1[q] = all[q];
+#pragma unroll
+  for (int q = 0; q < K2; ++q) o2[q] = all[K1 + q];
 }
 
 __device__ __forceinline__ bool finite(double v) { return __builtin_isfinite(v); }

@@ -34,7 +34,6 @@ hipError_t launch_bnorm(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_save_best(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_zero_slot(int prec64, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_fin(int kind, const SysDesc*, const int* syslist, int nsys, hipStream_t);
-enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3 };
 hipError_t launch_diag(const AxisCoef<double>*, int, int, int, const void*, void*, double, double, int, hipStream_t);
 }  // namespace mx
 
@@ -107,6 +106,13 @@ int refresh_iters(int stall_iters) {
   return stall_iters > 0 ? std::min(stall_iters, kRefreshIters) : kRefreshIters;
 }
 
+// One finishing launch per fused iteration (default); MEANAS_AMD_MERGED_FIN=0 launches
+// the post-apply and post-update finishing kernels separately (same results).
+bool merged_fin_enabled() {
+  const char* v = getenv("MEANAS_AMD_MERGED_FIN");
+  return !(v && v[0] == '0');
+}
+
 int auto_slab(const mx_operator& op, int ty) {
   // enough blocks to fill 256 CUs several times over, but deep slabs to amortise the
   // two extra planes a slab reads; never more partials than one finishing block sums
@@ -133,9 +139,10 @@ int vec_blocks(long long n3) {
 
 size_t elem_bytes(int mode) { return mode == MX_MIXED ? 8 : 16; }
 
-// workspace layout per system: r, p, q, x_lo (inner precision), x_best (FP64), state, partials
+// workspace layout per system: r, p, q, x_lo (inner precision), x_best (FP64), state,
+// partials (apply and reductions), the update's partials
 struct WorkLayout {
-  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, total;
+  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, off_partial_vec, total;
 };
 
 WorkLayout work_layout(const mx_operator& op, int mode) {
@@ -150,7 +157,8 @@ WorkLayout work_layout(const mx_operator& op, int mode) {
   w.off_state = w.off_xbest + align_up((size_t)n3 * 16);
   w.off_partial = w.off_state + align_up(sizeof(SysState));
   const size_t nbmax = (size_t)std::max(apply_blocks_upper(op), kVecMaxBlocks);
-  w.total = w.off_partial + align_up(4 * nbmax * sizeof(double));
+  w.off_partial_vec = w.off_partial + align_up(4 * nbmax * sizeof(double));
+  w.total = w.off_partial_vec + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
   return w;
 }
 
@@ -194,6 +202,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   int graph_chunk = 0;
   bool classic_launch = true;   // enqueue the classic-form direction update every iteration
+  bool merged_fin = false;      // every system stays fused: one finishing launch per iteration
 
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -244,6 +253,7 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.vec[VS_XBEST] = wk + w.off_xbest;
     d.st = reinterpret_cast<SysState*>(wk + w.off_state);
     d.partial = reinterpret_cast<double*>(wk + w.off_partial);
+    d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
     B.h_sys[s] = d;
     for (int b = 0; b < d.nblk_apply; ++b) nat_apply.push_back({s, b});
     for (int b = 0; b < d.nblk_vec; ++b) nat_vec.push_back({s, b});
@@ -300,6 +310,12 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
+    if (B.merged_fin) {   // the previous update is finished together with this apply
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+      MX_TRY(launch_fin(FIN_APPLY_MERGED, B.d_sys, B.d_syslist, B.nsl, s));
+      MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
+      return 0;
+    }
     if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
@@ -328,8 +344,12 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
 int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
   for (int it = 0; it < iters; ++it) {
     if (enqueue_iteration(B, s)) return -1;
-    if ((it + 1) % check_every == 0 || it + 1 == iters)
+    if ((it + 1) % check_every == 0 || it + 1 == iters) {
+      // merged launches: finish the last update, so that the state the replacement and
+      // the host read is current
+      if (B.merged_fin) MX_TRY(launch_fin(FIN_UPDATE, B.d_sys, B.d_syslist, B.nsl, s));
       if (enqueue_replacement(B, s)) return -1;
+    }
   }
   return 0;
 }
@@ -615,6 +635,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   Batch B;
   if (build_batch(B, ops, nops, systems, nsys, o)) return -1;
   B.classic_launch = o.form != MX_FORM_FUSED;
+  B.merged_fin = !B.classic_launch && merged_fin_enabled();
   hipStream_t user = reinterpret_cast<hipStream_t>(stream);
   hipStream_t s = B.stream;
   // order after the caller's pending work (inputs are produced on the caller's stream)
@@ -923,6 +944,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   d.vec[VS_XBEST] = wk + w.off_xbest;
   d.st = reinterpret_cast<SysState*>(wk + w.off_state);
   d.partial = reinterpret_cast<double*>(wk + w.off_partial);
+  d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
   S->h_sys[slot] = d;
   S->h_ops[slot] = *op;
 

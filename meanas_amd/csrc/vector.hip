@@ -17,7 +17,11 @@
 // q^T q = alpha^2 q^T q - rho.  Knowing beta before r' is reduced lets the direction
 // update ride in the same pass as x and r: 9 vector streams per iteration instead of
 // 11 (Chronopoulos & Gear 1989 use the same identity).  The exact rho' still comes
-// from the stored r' and is what alpha uses.
+// from the stored r' and is what alpha uses.  Batches whose systems all stay fused
+// (mx_solve in complex128) launch k_fin<1> only before a replacement point: in between,
+// k_fin<4> finishes the previous update and the apply together, so an iteration is
+// k_apply, k_fin<4>, k_update.  The state evolves exactly as with separate launches; an
+// apply that follows the update which paused its system is discarded.
 //
 // Classic form (complex64 inner vectors; complex128 systems whose recursion lost track):
 // the prediction above cancels (|r^T r| is often 1e-3 ||r||^2 for these complex-symmetric
@@ -39,9 +43,9 @@
 // the next k_pupdate.  A fused system whose true residual at a replacement exceeds 10x
 // the recursive one switches to the classic form (with a restart): in place, or (with
 // SF_SWITCH_EXIT, as mx_solve runs it) by finishing so that the host continues it in a
-// classic-form solve; then the fused iteration needs no direction-update launch.  Every scalar is reduced in a fixed order (per-block partials,
-// then one finishing block per system), so results do not depend on block scheduling
-// or on the batch.
+// classic-form solve; then the fused iteration needs no direction-update launch.  Every
+// scalar is reduced in a fixed order (per-block partials, then one finishing block per
+// system), so results do not depend on block scheduling or on the batch.
 #include "common.hpp"
 
 namespace mx {
@@ -211,7 +215,7 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm)
   }
   double acc[3] = {(double)rr_re, (double)rr_im, (double)nr};
   block_sum<3>(acc, s_red);
-  store_partials<3>(acc, S.partial, bm.local, S.nblk_vec);
+  store_partials<3>(acc, S.partial_vec, bm.local, S.nblk_vec);
 }
 
 template <typename T>
@@ -371,104 +375,131 @@ __global__ __launch_bounds__(VB) void k_diag(const AxisCoef<double>* __restrict_
 // ---- finishing kernels: one block per system sums that system's partials in a fixed
 // order and advances its scalar state.  They follow the reducing kernel of the same
 // kind and apply the same skip rule, so partials are never read stale.
-enum : int { FIN_APPLY = 0, FIN_UPDATE = 1, FIN_RESID = 2, FIN_BNORM = 3 };
+// FIN_APPLY_MERGED (fused FP64 batches) first finishes the previous update, whose
+// partials sit in their own array, then the apply: one launch per iteration instead of
+// two.  The apply that ran in between is discarded when that update pauses the system.
+
+// alpha (and the fused form's beta) from p^T A p and q^T q; tot = {mu re, im, qtq re, im}
+// ({., ., ||A g||^2, .} for a rescue step)
+__device__ __forceinline__ void fin_apply_state(SysState* st, int& f, const double (&tot)[4], bool fused) {
+  if (f & SF_MR) {
+    // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
+    // new residual r - alpha A g generic (r^T r no longer ~ 0); beta = 0 restarts
+    // COCG from that residual.
+    const double qq = tot[2];
+    const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
+    if (!(qq > 0.0) || !finite(alpha.re)) f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0
+    else {
+      st->alpha = alpha;
+      if (fused) {   // beta = 0 restarts the fused recurrence (classic: k_pupdate does);
+        st->beta = cplx<double>{0.0, 0.0};   // the step is taken, so the next apply is
+        f &= ~(SF_RESTART | SF_MR);           // an ordinary one
+      }
+    }
+  } else {
+    const cplx<double> mu{tot[0], tot[1]}, qtq{tot[2], tot[3]};
+    st->mu = mu;
+    const cplx<double> rho = st->rho;
+    const cplx<double> alpha = cdiv(rho, mu);
+    if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
+      // p^T A p vanished (or overflowed): replace the residual, then rescue with a
+      // step along a hashed direction and restart from p = r
+      f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
+      if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
+    } else {
+      if (fused) {   // predicted rho' = alpha^2 q^T q - rho; beta = rho' / rho
+        cplx<double> beta = cdiv(alpha * alpha * qtq, rho) - cplx<double>{1.0, 0.0};
+        if (!finite(beta.re) || !finite(beta.im)) beta = cplx<double>{0.0, 0.0};
+        st->beta = beta;
+      }
+      st->alpha = alpha;
+      f &= ~SF_RESTART;
+    }
+  }
+  if (!(f & SF_PAUSE)) f |= SF_UPD_PENDING;   // the update runs next
+}
+
+// rho = r^T r, ||r||^2 of the updated residual and the checks; tot = {rho re, im, ||r||^2}
+__device__ __forceinline__ void fin_update_state(SysState* st, int& f, const double (&tot)[3], bool fused) {
+  const cplx<double> rho{tot[0], tot[1]};
+  const double rn2 = tot[2];
+  f &= ~(SF_SAVE | SF_UPD_PENDING);   // any saved best iterate was stored by k_save_best before this
+  const bool rescued = (f & SF_MR) != 0;
+  f &= ~SF_MR;       // a rescue step (if that was one) is done
+  if (!fused) {      // classic: beta from the exact rho'; a rescue step restarts
+    if (rescued) st->beta = cplx<double>{0.0, 0.0};
+    else if (norm2(st->rho) > 0.0) st->beta = cdiv(rho, st->rho);
+    else {           // rho breakdown: restart from the replaced residual
+      f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
+      st->beta = cplx<double>{0.0, 0.0};
+    }
+  }
+  // r^T r ~ 0 with r != 0: the next step is a rescue step along a hashed direction.
+  // The fused form has already built p from r, so it pauses for a replacement, after
+  // which k_pupdate writes the rescue direction.
+  if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0)
+    f |= SF_MR | SF_RESTART | (fused ? SF_PAUSE : 0);
+  st->rho_old = st->rho;
+  st->rho = rho;
+  st->rnorm2 = rn2;
+  if (rn2 > st->rmax2) st->rmax2 = rn2;
+  st->iters += 1;
+  st->since += 1;
+  if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
+    f |= SF_NAN | SF_PAUSE | SF_LIMIT;
+  } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
+             (st->refresh_iters > 0 && st->since >= st->refresh_iters)) {
+    f |= SF_PAUSE;
+  }
+  if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
+}
 
 template <int KIND>
 __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
-  __shared__ double s_red[4 * 16];
+  __shared__ double s_red[8 * 16];
   const SysDesc& S = systems[syslist[blockIdx.x]];
   SysState* st = S.st;
   int f = st->flags;
-  if (KIND == FIN_APPLY || KIND == FIN_UPDATE) {
+  if (KIND == FIN_APPLY || KIND == FIN_APPLY_MERGED) {
     if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  } else if (KIND == FIN_UPDATE) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, SF_UPD_PENDING)) return;
   } else if (KIND == FIN_RESID) {
     if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
   } else {
     if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
   }
   const bool fused = (f & SF_FUSED) != 0;
-  const int nblk = (KIND == FIN_APPLY || KIND == FIN_RESID) ? S.nblk_apply : S.nblk_vec;
+  if constexpr (KIND == FIN_APPLY_MERGED) {
+    const bool pending = (f & SF_UPD_PENDING) != 0;
+    double tu[3], ta[4];
+    final_sum2<3, 4>(tu, S.partial_vec, pending ? S.nblk_vec : 0, ta, S.partial, S.nblk_apply, s_red);
+    if (threadIdx.x != 0) return;
+    if (pending) fin_update_state(st, f, tu, fused);
+    if (!(f & SF_PAUSE)) fin_apply_state(st, f, ta, fused);
+    st->flags = f;
+    return;
+  }
   double tot[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (KIND == FIN_BNORM) {
     double t1[1];
-    final_sum<1>(t1, S.partial, nblk, s_red);
+    final_sum<1>(t1, S.partial, S.nblk_vec, s_red);
     tot[0] = t1[0];
   } else if constexpr (KIND == FIN_APPLY) {
-    final_sum<4>(tot, S.partial, nblk, s_red);
+    final_sum<4>(tot, S.partial, S.nblk_apply, s_red);
   } else {
     double t3[3];
-    final_sum<3>(t3, S.partial, nblk, s_red);
+    final_sum<3>(t3, KIND == FIN_UPDATE ? S.partial_vec : S.partial, KIND == FIN_UPDATE ? S.nblk_vec : S.nblk_apply,
+                 s_red);
     tot[0] = t3[0]; tot[1] = t3[1]; tot[2] = t3[2];
   }
   if (threadIdx.x != 0) return;
 
   if constexpr (KIND == FIN_APPLY) {
-    if (f & SF_MR) {
-      // rescue step along the hashed direction p = g: alpha = ||r|| / ||A g|| makes the
-      // new residual r - alpha A g generic (r^T r no longer ~ 0); beta = 0 restarts
-      // COCG from that residual.
-      const double qq = tot[2];
-      const cplx<double> alpha{sqrt(st->rnorm2 / qq), 0.0};
-      if (!(qq > 0.0) || !finite(alpha.re)) f |= SF_BREAKDOWN | SF_PAUSE | SF_LIMIT;   // A g = 0
-      else {
-        st->alpha = alpha;
-        if (fused) {   // beta = 0 restarts the fused recurrence (classic: k_pupdate does)
-          st->beta = cplx<double>{0.0, 0.0};
-          f &= ~SF_RESTART;
-        }
-      }
-    } else {
-      const cplx<double> mu{tot[0], tot[1]}, qtq{tot[2], tot[3]};
-      st->mu = mu;
-      const cplx<double> rho = st->rho;
-      const cplx<double> alpha = cdiv(rho, mu);
-      if (!(norm2(mu) > 0.0) || !finite(alpha.re) || !finite(alpha.im)) {
-        // p^T A p vanished (or overflowed): replace the residual, then rescue with a
-        // step along a hashed direction and restart from p = r
-        f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART | SF_MR;
-        if (!finite(mu.re) || !finite(mu.im)) f |= SF_NAN | SF_LIMIT;
-      } else {
-        if (fused) {   // predicted rho' = alpha^2 q^T q - rho; beta = rho' / rho
-          cplx<double> beta = cdiv(alpha * alpha * qtq, rho) - cplx<double>{1.0, 0.0};
-          if (!finite(beta.re) || !finite(beta.im)) beta = cplx<double>{0.0, 0.0};
-          st->beta = beta;
-        }
-        st->alpha = alpha;
-        f &= ~SF_RESTART;
-      }
-    }
+    fin_apply_state(st, f, tot, fused);
   } else if constexpr (KIND == FIN_UPDATE) {
-    const cplx<double> rho{tot[0], tot[1]};
-    const double rn2 = tot[2];
-    f &= ~SF_SAVE;     // any saved best iterate was stored by k_save_best before this
-    const bool rescued = (f & SF_MR) != 0;
-    f &= ~SF_MR;       // a rescue step (if that was one) is done
-    if (!fused) {      // classic: beta from the exact rho'; a rescue step restarts
-      if (rescued) st->beta = cplx<double>{0.0, 0.0};
-      else if (norm2(st->rho) > 0.0) st->beta = cdiv(rho, st->rho);
-      else {           // rho breakdown: restart from the replaced residual
-        f |= SF_BREAKDOWN | SF_PAUSE | SF_RESTART;
-        st->beta = cplx<double>{0.0, 0.0};
-      }
-    }
-    // r^T r ~ 0 with r != 0: the next step is a rescue step along a hashed direction.
-    // The fused form has already built p from r, so it pauses for a replacement, after
-    // which k_pupdate writes the rescue direction.
-    if (norm2(rho) <= kRhoBreakdown * kRhoBreakdown * rn2 * rn2 && rn2 > 0.0)
-      f |= SF_MR | SF_RESTART | (fused ? SF_PAUSE : 0);
-    st->rho_old = st->rho;
-    st->rho = rho;
-    st->rnorm2 = rn2;
-    if (rn2 > st->rmax2) st->rmax2 = rn2;
-    st->iters += 1;
-    st->since += 1;
-    if (!finite(rn2) || !finite(rho.re) || !finite(rho.im)) {
-      f |= SF_NAN | SF_PAUSE | SF_LIMIT;
-    } else if (rn2 <= st->stop2 || rn2 <= st->delta2 * st->rmax2 ||
-               (st->refresh_iters > 0 && st->since >= st->refresh_iters)) {
-      f |= SF_PAUSE;
-    }
-    if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
+    const double t3[3] = {tot[0], tot[1], tot[2]};
+    fin_update_state(st, f, t3, fused);
   } else if constexpr (KIND == FIN_RESID) {
     // residual replacement / initial residual
     const double rn2 = tot[0];
@@ -561,6 +592,9 @@ hipError_t launch_fin(int kind, const SysDesc* sys, const int* syslist, int nsys
     case FIN_APPLY: hipLaunchKernelGGL(k_fin<FIN_APPLY>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
     case FIN_UPDATE: hipLaunchKernelGGL(k_fin<FIN_UPDATE>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
     case FIN_RESID: hipLaunchKernelGGL(k_fin<FIN_RESID>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
+    case FIN_APPLY_MERGED:
+      hipLaunchKernelGGL(k_fin<FIN_APPLY_MERGED>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist);
+      break;
     default: hipLaunchKernelGGL(k_fin<FIN_BNORM>, dim3(nsys), dim3(kFinThreads), 0, s, sys, syslist); break;
   }
   return hipGetLastError();

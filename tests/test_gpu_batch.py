@@ -5,6 +5,7 @@
     the single-system solve bit for bit;
   * an uneven batch of structures (different grid sizes, PEC blocks): every solution
     meets the physical residual target with the oracle's operator;
+  * one finishing launch per fused iteration gives the same bits as two;
   * the continuous-batching server: each result is the same alone and in the mix
     (bit for bit), forward and adjoint requests, with admission while others iterate.
 """
@@ -52,6 +53,43 @@ def test_multi_rhs_batch(method):
     x1 = torch.zeros_like(bs[0])
     solve_batch([op], [dict(op=0, b=bs[0], x=x1, rtol=1e-9)], mode=mode)
     assert torch.equal(x1, xs[0])
+
+
+@pytest.mark.parametrize('case', ['aniso_batch', 'near_static'])
+def test_merged_finishing_equals_separate(case, monkeypatch):
+    """Fused batches finish the update and the next apply in one launch; separate
+    launches (MEANAS_AMD_MERGED_FIN=0) must give the same bits and statistics."""
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    if case == 'aniso_batch':
+        omega, dxes, eps, Js = small_aniso(nrhs=3)
+        rtol = 1e-9
+    else:   # the reference's near-static fixture (meanas/test/test_fdfd.py): fused
+        shape = (5, 5, 5)      # systems lose track and finish for the classic re-solve
+        omega = 1 / 1500
+        eps = numpy.ones((3, *shape))
+        eps[:, 2, 2, 2] = 2.5
+        dxes = [[numpy.full(5, 1.5) for _ in range(3)] for _ in range(2)]
+        J = numpy.zeros((3, *shape), dtype=complex)
+        J[:, 3, 3, 3] = 1 + 1j
+        J[:, 1, 1, 1] = 1 - 1j
+        eps, Js, rtol = eps.ravel(), [J.ravel()], 1e-11
+    op = DeviceOperator(omega, dxes, eps)
+    bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega) for J in Js]
+    runs = []
+    for merged in ('1', '0'):
+        monkeypatch.setenv('MEANAS_AMD_MERGED_FIN', merged)
+        xs = [torch.zeros_like(b) for b in bs]
+        st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol) for b, x in zip(bs, xs)], mode=_lib.MX_FP64,
+                         options=dict(form='fused'))
+        runs.append((xs, st))
+    (xm, sm), (xs_, ss) = runs
+    for a, b in zip(xm, xs_):
+        assert torch.equal(a, b)
+    assert sm == ss
+    if case == 'aniso_batch':   # the near-static system reaches the iteration cap, identically
+        assert all(s['status'] == 'converged' for s in sm), sm
 
 
 def test_uneven_structures_with_pec():
