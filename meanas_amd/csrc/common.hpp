@@ -108,6 +108,13 @@ __device__ __forceinline__ typename PackV<T>::type pload(const void* base, long 
   const unsigned off = (unsigned)i * 16u;
   return *(const MX_GLOBAL V*)((const MX_GLOBAL char*)base + off);
 }
+// streaming load: for data not read again before much other traffic has passed
+template <typename T>
+__device__ __forceinline__ typename PackV<T>::type pload_nt(const void* base, long long i) {
+  typedef typename PackV<T>::type V;
+  const unsigned off = (unsigned)i * 16u;
+  return __builtin_nontemporal_load((const MX_GLOBAL V*)((const MX_GLOBAL char*)base + off));
+}
 template <typename T>
 __device__ __forceinline__ void pstore(void* base, long long i, typename PackV<T>::type v) {
   typedef typename PackV<T>::type V;
