@@ -214,6 +214,33 @@ int mx_fdtd_e(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, const voi
 int mx_fdtd_run(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, void* h, const void* eps,
                 double eps_scalar, const void* mu, double mu_scalar, int32_t steps, void* stream);
 
+/* Convolutional PML (fdtd/pml.py).  One layer covers cells [0, thickness) (polarity -1) or
+ * [n - thickness, n) (polarity +1) along `axis`.  p_e / p_h: device float64 arrays of
+ * 3 x thickness values, the profiles p0, p1, p2 at the E and H sample points, indexed
+ * by the cell's position in the layer (pml.py:23-93, cpml_params).  psi_e / psi_h: device
+ * arrays of 2 x (cells of the layer) auxiliary values (psi_u then psi_v, each in the
+ * layer's C-order sub-grid), float32 or float64 (psi_prec), zeroed by the caller before
+ * the first step and kept between steps. */
+typedef struct mx_cpml_layer {
+  int32_t axis, polarity, thickness, reserved;
+  const double* p_e;
+  const double* p_h;
+  void* psi_e;
+  void* psi_h;
+} mx_cpml_layer;
+#define MX_CPML_MAX_LAYERS 6
+/* update_E of updates_with_cpml (fdtd/pml.py:139-183): e_c += dt / eps_c (curl_c + pE_c),
+ * the derivatives along each layer's axis scaled by p2 inside it, psi advanced with p0
+ * and p1.  Every difference uses g->dxh (pml.py:106-107; g->dxe is not read).  Layers in
+ * increasing (axis, polarity) order, at most one per face.  eps: 3N values or NULL for
+ * eps_scalar. */
+int mx_fdtd_cpml_e(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, void* e, const void* h,
+                   const void* eps, double eps_scalar, const mx_cpml_layer* layers, int32_t nlayers, void* stream);
+/* update_H of updates_with_cpml (fdtd/pml.py:185-229): h_c -= dt / mu_c (curl_c + pH_c).
+ * mu: 3N values or NULL for mu_scalar (the reference's default mu = (1, 1, 1)). */
+int mx_fdtd_cpml_h(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, const void* e, void* h,
+                   const void* mu, double mu_scalar, const mx_cpml_layer* layers, int32_t nlayers, void* stream);
+
 /* Kernel timing of solver iterations (measurement; no counterpart in the reference).
  * Sets up `sys` as mx_solve would (r = b - A x, p = r) with every stopping rule off, runs
  * `reps` iterations on `stream` exactly as a solve launches them, with a HIP event

@@ -82,6 +82,16 @@ class MxFdtdGrid(ctypes.Structure):
                 ('dxe', c_void_p * 3), ('dxh', c_void_p * 3)]
 
 
+class MxCpmlLayer(ctypes.Structure):
+    """mx_cpml_layer: one CPML face (profiles and psi state on the device)."""
+    _fields_ = [('axis', ctypes.c_int32), ('polarity', ctypes.c_int32), ('thickness', ctypes.c_int32),
+                ('reserved', ctypes.c_int32), ('p_e', c_void_p), ('p_h', c_void_p), ('psi_e', c_void_p),
+                ('psi_h', c_void_p)]
+
+
+MX_CPML_MAX_LAYERS = 6
+
+
 class MxStats(ctypes.Structure):
     _fields_ = [
         ('status', ctypes.c_int32), ('iterations', ctypes.c_int32),
@@ -121,6 +131,12 @@ PROTOTYPES = {
                                  c_void_p, ctypes.c_double, c_void_p]),
     'mx_fdtd_run': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_double, c_void_p, c_void_p,
                                    c_void_p, ctypes.c_double, c_void_p, ctypes.c_double, ctypes.c_int32, c_void_p]),
+    'mx_fdtd_cpml_e': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                      c_void_p, c_void_p, c_void_p, ctypes.c_double, ctypes.POINTER(MxCpmlLayer),
+                                      ctypes.c_int32, c_void_p]),
+    'mx_fdtd_cpml_h': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                      c_void_p, c_void_p, c_void_p, ctypes.c_double, ctypes.POINTER(MxCpmlLayer),
+                                      ctypes.c_int32, c_void_p]),
     'mx_time_kernels': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), ctypes.POINTER(MxOptions),
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, c_void_p]),
     'mx_session_create': (c_void_p, [ctypes.c_int32, ctypes.POINTER(MxOptions)]),

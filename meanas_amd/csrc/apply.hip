@@ -68,7 +68,10 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 // Occupancy target (the second launch-bounds argument is the minimum number of waves
 // per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs about 190 VGPRs,
 // and holding it to 3 waves (168) spills, so it runs 2.
-template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : 2;
+#ifndef MX_APPLY_WAVES64
+#define MX_APPLY_WAVES64 2
+#endif
+template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : MX_APPLY_WAVES64;
 
 template <typename T, int TY, int EPI, typename RT, int PD>
 __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpView<T>* __restrict__ ops,

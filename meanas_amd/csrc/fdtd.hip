@@ -77,6 +77,91 @@ __global__ __launch_bounds__(kFB) void k_fdtd_e(RGrid g, T dt, T* __restrict__ e
   }
 }
 
+// ---- convolutional PML (fdtd/pml.py:96-230) ---------------------------------------
+// One thread per cell.  The six derivatives d_a f_c (a != c) use dx_h along every axis.
+// Each layer the cell lies in (in the table's order: axis 0..2, polarity -1 then +1)
+// scales the two derivatives along its axis by p2, advances the cell's psi_u, psi_v
+// (u < v the components other than the axis) with p0 and p1, stores them in the psi
+// precision, and adds s psi_u to the auxiliary term of u and -s psi_v to v's, in the psi
+// precision, s = (-1, 1, -1)[axis].  Precisions follow the reference's array code with
+// float64 widths: differences of field values in the field precision, divided by the
+// float64 width, so derivatives, psi updates and the final sum are double; dt / material
+// is taken in the material's (field) precision, dt / scalar in double.
+struct CpmlLayer {
+  int axis, start, thickness;
+  const double* pe;
+  const double* ph;
+  void* psie;
+  void* psih;
+};
+struct CpmlTable {
+  int n;
+  CpmlLayer l[MX_CPML_MAX_LAYERS];
+};
+
+template <typename T, typename TP, bool EUPD>
+__global__ __launch_bounds__(kFB) void k_cpml(RGrid g, T dt, T* __restrict__ e, T* __restrict__ h,
+                                              const T* __restrict__ mat, double mat_s, double dt64, CpmlTable tab) {
+  const int nc = g.nx * g.ny * g.nz;
+  const int dims[3] = {g.nx, g.ny, g.nz};
+  const int strides[3] = {g.ny * g.nz, g.nz, 1};
+  const T* f = EUPD ? h : e;
+  T* out = EUPD ? e : h;
+  for (int cell = blockIdx.x * kFB + threadIdx.x; cell < nc; cell += gridDim.x * kFB) {
+    const int k = cell % g.nz, t = cell / g.nz, j = t % g.ny, i = t / g.ny;
+    const int idx[3] = {i, j, k};
+    // d[a][c]: derivative along a of component c (E update: backward of h; H: forward of e)
+    double d[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double w = width(g.dxh[a], idx[a]);
+      const int s = EUPD ? (idx[a] == 0 ? (dims[a] - 1) * strides[a] : -strides[a])
+                         : (idx[a] + 1 == dims[a] ? -idx[a] * strides[a] : strides[a]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (c == a) continue;
+        const T here = f[c * nc + cell], there = f[c * nc + cell + s];
+        d[a][c] = (double)(EUPD ? here - there : there - here) / w;
+      }
+    }
+    TP aux[3] = {TP(0), TP(0), TP(0)};
+    for (int q = 0; q < tab.n; ++q) {
+      const CpmlLayer& L = tab.l[q];
+      const int a = L.axis;
+      const int pos = idx[a] - L.start;
+      if (pos < 0 || pos >= L.thickness) continue;
+      const int u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+      const double* p = EUPD ? L.pe : L.ph;
+      const double p0 = p[pos], p1 = p[L.thickness + pos], p2 = p[2 * L.thickness + pos];
+      d[a][u] *= p2;
+      d[a][v] *= p2;
+      // index in the layer's C-order sub-grid (its axis `thickness` cells long)
+      int sub[3] = {i, j, k}, sd[3] = {g.nx, g.ny, g.nz};
+      sub[a] = pos;
+      sd[a] = L.thickness;
+      const int r = (sub[0] * sd[1] + sub[1]) * sd[2] + sub[2];
+      const int rn = sd[0] * sd[1] * sd[2];
+      TP* psi = reinterpret_cast<TP*>(EUPD ? L.psie : L.psih);
+      TP pu = (TP)((double)psi[r] * p0), pv = (TP)((double)psi[rn + r] * p0);
+      pu = (TP)((double)pu + p1 * d[a][v]);
+      pv = (TP)((double)pv + p1 * d[a][u]);
+      psi[r] = pu;
+      psi[rn + r] = pv;
+      const TP sg = a == 1 ? TP(1) : TP(-1);
+      aux[u] = aux[u] + sg * pu;
+      aux[v] = aux[v] + (-sg) * pv;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      const double curl = (d[c1][c2] - d[c2][c1]) + (double)aux[c];
+      const double scale = mat ? (double)(dt / mat[c * nc + cell]) : dt64 / mat_s;
+      const double o = (double)out[c * nc + cell];
+      out[c * nc + cell] = (T)(EUPD ? o + scale * curl : o - scale * curl);
+    }
+  }
+}
+
 bool rgrid(const mx_fdtd_grid* g, RGrid* v, std::string* why) {
   if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) { *why = "bad grid"; return false; }
   if (3LL * g->nx * g->ny * g->nz >= (1LL << 31)) { *why = "grid too large (3N must be < 2^31)"; return false; }
@@ -113,6 +198,46 @@ hipError_t launch_e(const RGrid& v, int prec, double dt, void* e, const void* h,
   else
     hipLaunchKernelGGL(k_fdtd_e<float>, dim3(blocks(v)), dim3(kFB), 0, s, v, (float)dt, (float*)e, (const float*)h,
                        (const float*)eps, (float)eps_s);
+  return hipGetLastError();
+}
+
+// Layer table of a CPML update: faces in increasing (axis, polarity) order (the order
+// the reference accumulates in), each inside the grid, with its arrays.
+bool cpml_table(const RGrid& v, const mx_cpml_layer* layers, int n, CpmlTable* tab, std::string* why) {
+  if (n < 0 || n > MX_CPML_MAX_LAYERS) { *why = "nlayers must be 0..6"; return false; }
+  if (n > 0 && !layers) { *why = "null layer table"; return false; }
+  const int dims[3] = {v.nx, v.ny, v.nz};
+  tab->n = n;
+  int prev = -1;
+  for (int q = 0; q < n; ++q) {
+    const mx_cpml_layer& L = layers[q];
+    if (L.axis < 0 || L.axis > 2) { *why = "layer axis must be 0, 1 or 2"; return false; }
+    if (L.polarity != -1 && L.polarity != 1) { *why = "layer polarity must be -1 or +1"; return false; }
+    if (L.thickness < 1 || L.thickness > dims[L.axis]) { *why = "layer thickness must be 1..n along its axis"; return false; }
+    const int key = 2 * L.axis + (L.polarity > 0);
+    if (key <= prev) { *why = "layers must be in increasing (axis, polarity) order, one per face"; return false; }
+    prev = key;
+    if (!L.p_e || !L.p_h || !L.psi_e || !L.psi_h) { *why = "null layer array"; return false; }
+    tab->l[q] = CpmlLayer{L.axis, L.polarity < 0 ? 0 : dims[L.axis] - L.thickness, L.thickness, L.p_e, L.p_h,
+                          L.psi_e, L.psi_h};
+  }
+  return true;
+}
+
+template <bool EUPD>
+hipError_t launch_cpml(const RGrid& v, int prec, int psi_prec, double dt, const void* e, const void* h,
+                       const void* mat, double mat_s, const CpmlTable& tab, hipStream_t s) {
+#define MX_CPML_CASE(T, TP)                                                                                  \
+  hipLaunchKernelGGL((k_cpml<T, TP, EUPD>), dim3(blocks(v)), dim3(kFB), 0, s, v, (T)dt, (T*)e, (T*)h,       \
+                     (const T*)mat, mat_s, dt, tab)
+  if (prec == MX_REAL64) {
+    if (psi_prec == MX_REAL64) MX_CPML_CASE(double, double);
+    else MX_CPML_CASE(double, float);
+  } else {
+    if (psi_prec == MX_REAL64) MX_CPML_CASE(float, double);
+    else MX_CPML_CASE(float, float);
+  }
+#undef MX_CPML_CASE
   return hipGetLastError();
 }
 
@@ -157,6 +282,32 @@ int mx_fdtd_run(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, void* h
     if (err != hipSuccess) return mx::set_error(std::string("mx_fdtd_run: ") + hipGetErrorString(err));
   }
   return 0;
+}
+
+int mx_fdtd_cpml_e(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, void* e, const void* h,
+                   const void* eps, double eps_scalar, const mx_cpml_layer* layers, int32_t nlayers, void* stream) {
+  mx::RGrid v;
+  if (mx::check_args(g, prec, e, h, &v, "mx_fdtd_cpml_e")) return -1;
+  if (psi_prec != MX_REAL32 && psi_prec != MX_REAL64) return mx::set_error("mx_fdtd_cpml_e: psi_prec must be MX_REAL32 or MX_REAL64");
+  mx::CpmlTable tab;
+  std::string why;
+  if (!mx::cpml_table(v, layers, nlayers, &tab, &why)) return mx::set_error("mx_fdtd_cpml_e: " + why);
+  const hipError_t err = mx::launch_cpml<true>(v, prec, psi_prec, dt, e, h, eps, eps_scalar, tab,
+                                               reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_cpml_e: ") + hipGetErrorString(err));
+}
+
+int mx_fdtd_cpml_h(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, const void* e, void* h,
+                   const void* mu, double mu_scalar, const mx_cpml_layer* layers, int32_t nlayers, void* stream) {
+  mx::RGrid v;
+  if (mx::check_args(g, prec, e, h, &v, "mx_fdtd_cpml_h")) return -1;
+  if (psi_prec != MX_REAL32 && psi_prec != MX_REAL64) return mx::set_error("mx_fdtd_cpml_h: psi_prec must be MX_REAL32 or MX_REAL64");
+  mx::CpmlTable tab;
+  std::string why;
+  if (!mx::cpml_table(v, layers, nlayers, &tab, &why)) return mx::set_error("mx_fdtd_cpml_h: " + why);
+  const hipError_t err = mx::launch_cpml<false>(v, prec, psi_prec, dt, e, h, mu, mu_scalar, tab,
+                                                reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_cpml_h: ") + hipGetErrorString(err));
 }
 
 }  // extern "C"

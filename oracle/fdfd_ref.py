@@ -553,3 +553,109 @@ def delta_energy_j(j0, e1, dxes=None):
     if dxes is None:
         dxes = [[numpy.ones(1)] * 3] * 2
     return (j0 * e1).sum(axis=0) * _grid_weights(dxes[0])
+
+
+# ---------------------------------------------------------------------------
+# FDTD convolutional PML  (meanas/fdtd/pml.py:23-230)
+# ---------------------------------------------------------------------------
+def cpml_params(axis: int, polarity: int, dt: float, thickness: int = 8, ln_R_per_layer: float = -1.6,
+                epsilon_eff: float = 1, mu_eff: float = 1, m: float = 3.5, ma: float = 1,
+                cfs_alpha: float = 0) -> dict:
+    """CPML profiles of one layer (pml.py:23-93).
+
+    Slab index l = 0..t-1 (cells [0, t) for polarity -1, [n - t, n) for +1).  Depth x of
+    the sample points, in cells from the inner face: polarity +1 has E at l + 1/2 and H
+    at l + 1; polarity -1 mirrors them (E at t - l, H at t - l - 1/2).  With s = (x/t)^m:
+    sigma = s sigma_max, kappa = 1 + s (kappa_max - 1), alpha = (1 - x/t)^ma alpha_max,
+    p0 = exp(-(sigma/kappa + alpha) dt), p1 = sigma / (sigma + kappa alpha) (p0 - 1),
+    p2 = 1/kappa; sigma_max = -ln_R_per_layer (m + 1) / 2, kappa_max = sqrt(eps_eff mu_eff).
+    """
+    if axis not in (0, 1, 2):
+        raise ValueError(f'Invalid axis: {axis}')
+    if polarity not in (-1, 1):
+        raise ValueError(f'Invalid polarity: {polarity}')
+    if thickness <= 2:
+        raise ValueError('It would be wise to have a pml with 4+ cells of thickness')
+    if epsilon_eff <= 0:
+        raise ValueError('epsilon_eff must be positive')
+    t = thickness
+    l = numpy.arange(t, dtype=float)
+    if polarity > 0:
+        xe, xh = l + 0.5, l + 1.0
+    else:
+        xe, xh = t - l, t - l - 0.5
+    sigma_max = -ln_R_per_layer / 2 * (m + 1)
+    kappa_max = numpy.sqrt(epsilon_eff * mu_eff)
+    shape = [1, 1, 1]
+    shape[axis] = t
+
+    def profiles(x):
+        s = (x / t) ** m
+        sigma = sigma_max * s
+        kappa = 1 + (kappa_max - 1) * s
+        alpha = cfs_alpha * (1 - x / t) ** ma
+        p0 = numpy.exp(-(sigma / kappa + alpha) * dt)
+        p1 = sigma / (sigma + kappa * alpha) * (p0 - 1)
+        return tuple(p.reshape(shape) for p in (p0, p1, 1 / kappa))
+
+    region = [slice(None)] * 3
+    region[axis] = slice(None, t) if polarity < 0 else slice(-t, None)
+    return {'param_e': profiles(xe), 'param_h': profiles(xh), 'region': tuple(region)}
+
+
+def updates_with_cpml(cpml_params, dt: float, dxes, epsilon, *, dtype=numpy.float32):
+    """(update_E(e, h, epsilon), update_H(e, h, mu=(1, 1, 1))) with CPML layers
+    (pml.py:96-230).  cpml_params[axis][p] (p = 0: polarity -1, 1: +1) is a layer dict or
+    None.  Both difference directions use dxes[1] (pml.py:106-107).  Per update the six
+    derivatives d_a f_c (a != c) are formed; each layer on axis a, in the order axis 0..2,
+    polarity -1 then +1, scales d_a f_u and d_a f_v (u < v the components other than a) by
+    p2 inside its region, then advances psi_u = p0 psi_u + p1 d_a f_v and psi_v = p0 psi_v +
+    p1 d_a f_u (psi in `dtype`), and adds s psi_u to the auxiliary term of component u and
+    -s psi_v to that of v, s = (-1, 1, -1)[a].  Then e_c += dt / eps_c (curl_c + pE_c) and
+    h_c -= dt / mu_c (curl_c + pH_c) with the scaled derivatives."""
+    w = [numpy.asarray(dxes[1][a]).reshape([-1 if b == a else 1 for b in range(3)]) for a in range(3)]
+    layers = []
+    for axis in range(3):
+        for pp in range(2):
+            c = cpml_params[axis][pp]
+            if c is None:
+                continue
+            rshape = epsilon[0][c['region']].shape
+            layers.append(dict(axis=axis, region=c['region'], pe=c['param_e'], ph=c['param_h'],
+                               psi_e=[numpy.zeros(rshape, dtype=dtype) for _ in range(2)],
+                               psi_h=[numpy.zeros(rshape, dtype=dtype) for _ in range(2)]))
+    acc = numpy.empty_like(epsilon, dtype=dtype)
+
+    def absorb(d, key_p, key_psi):
+        acc.fill(0)
+        for lay in layers:
+            a = lay['axis']
+            u, v = [c for c in range(3) if c != a]
+            p0, p1, p2 = lay[key_p]
+            psi_u, psi_v = lay[key_psi]
+            reg = lay['region']
+            d[a, u][reg] *= p2
+            d[a, v][reg] *= p2
+            psi_u *= p0
+            psi_v *= p0
+            psi_u += p1 * d[a, v][reg]
+            psi_v += p1 * d[a, u][reg]
+            s = (-1, 1, -1)[a]
+            acc[u][reg] += s * psi_u
+            acc[v][reg] += -s * psi_v
+
+    def update_E(e, h, epsilon):
+        d = {(a, c): (h[c] - numpy.roll(h[c], 1, axis=a)) / w[a] for a in range(3) for c in range(3) if a != c}
+        absorb(d, 'pe', 'psi_e')
+        for c in range(3):
+            c1, c2 = (c + 1) % 3, (c + 2) % 3
+            e[c] += dt / epsilon[c] * (d[c1, c2] - d[c2, c1] + acc[c])
+
+    def update_H(e, h, mu=(1, 1, 1)):
+        d = {(a, c): (numpy.roll(e[c], -1, axis=a) - e[c]) / w[a] for a in range(3) for c in range(3) if a != c}
+        absorb(d, 'ph', 'psi_h')
+        for c in range(3):
+            c1, c2 = (c + 1) % 3, (c + 2) % 3
+            h[c] -= dt / mu[c] * (d[c1, c2] - d[c2, c1] + acc[c])
+
+    return update_E, update_H

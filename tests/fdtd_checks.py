@@ -155,3 +155,44 @@ def check_poynting_planes(sim):
 
 CHECKS = [check_initial_fields, check_initial_energy, check_energy_conservation,
           check_poynting_divergence, check_poynting_planes]
+
+
+# ---- CPML (meanas/fdtd/pml.py; the reference has no tests of it) ----------------------
+def cpml_layers(make, dt, faces, thickness, **kw):
+    """cpml_params[axis][p] list with a layer on each (axis, polarity) in `faces`."""
+    out = [[None, None] for _ in range(3)]
+    for axis, pol in faces:
+        out[axis][(pol + 1) // 2] = make(axis, pol, dt, thickness=thickness, **kw)
+    return out
+
+
+ALL_FACES = [(a, p) for a in range(3) for p in (-1, 1)]
+
+
+def pulse_energy(updates, axis, comp, n=160, thickness=12, steps=500, dt=0.5, layers=True, to_dev=None, to_host=None,
+                 make=None):
+    """A Gaussian pulse in e[comp] on an n-cell line along `axis` (unit widths, eps = 1),
+    stepped H then E `steps` times: (final, max) field energy relative to the initial one."""
+    shape = [1, 1, 1]
+    shape[axis] = n
+    shape = tuple(shape)
+    dxes = [[numpy.ones(s) for s in shape] for _ in range(2)]
+    eps = numpy.ones((3, *shape))
+    faces = [(axis, -1), (axis, 1)] if layers else []
+    uE, uH = updates(cpml_layers(make, dt, faces, thickness), dt, dxes, eps)
+    e = numpy.zeros((3, *shape))
+    h = numpy.zeros_like(e)
+    e[comp] = numpy.exp(-((numpy.arange(n).reshape(shape) - n / 2) / 5.0) ** 2)
+    u0 = (e ** 2).sum()
+    if to_dev is not None:
+        e, h, eps = to_dev(e), to_dev(h), to_dev(eps)
+    top = 0.0
+    for _ in range(steps):
+        uH(e, h)
+        uE(e, h, eps)
+        if to_host is not None and _ % 50:
+            continue
+        ee, hh = (e, h) if to_host is None else (to_host(e), to_host(h))
+        top = max(top, float((ee ** 2).sum() + (hh ** 2).sum()) / u0)
+    ee, hh = (e, h) if to_host is None else (to_host(e), to_host(h))
+    return float((ee ** 2).sum() + (hh ** 2).sum()) / u0, top

@@ -78,3 +78,98 @@ def test_unit_widths_scalar_materials_and_float32():
     assert e32.dtype == numpy.float32
     numpy.testing.assert_allclose(e32, eo, rtol=1e-4, atol=1e-4)
     numpy.testing.assert_allclose(h32, ho, rtol=1e-4, atol=1e-4)
+
+
+# ---- CPML (meanas/fdtd/pml.py) against the oracle's restatement ---------------------
+def _cpml_case(shape, faces, thickness, seed, fdt, with_mu):
+    rng = numpy.random.default_rng(seed)
+    e = rng.standard_normal((3, *shape)).astype(fdt)
+    h = rng.standard_normal((3, *shape)).astype(fdt)
+    eps = (1 + rng.random((3, *shape))).astype(fdt)
+    mu = (1 + rng.random((3, *shape))).astype(fdt) if with_mu else None
+    dxes = [[0.8 + 0.4 * rng.random(n) for n in shape] for _ in range(2)]
+    return e, h, eps, mu, dxes
+
+
+@pytest.mark.parametrize('fdt,pdt,tol', [(numpy.float64, numpy.float64, 1e-11), (numpy.float64, numpy.float32, 1e-5),
+                                         (numpy.float32, numpy.float32, 1e-4), (numpy.float32, numpy.float64, 1e-4)])
+@pytest.mark.parametrize('shape,faces,thickness', [
+    ((14, 12, 9), fc.ALL_FACES, 4),
+    ((10, 9, 7), fc.ALL_FACES, 4),                 # z layers overlap (2 * 4 > 7)
+    ((16, 1, 11), [(0, 1), (2, -1)], 5),           # two faces, one flat axis
+])
+@pytest.mark.parametrize('with_mu', [False, True])
+def test_cpml_updates_match_oracle(fdt, pdt, tol, shape, faces, thickness, with_mu):
+    from meanas_amd import fdtd
+    e, h, eps, mu, dxes = _cpml_case(shape, faces, thickness, 5, fdt, with_mu)
+    dt = 0.3
+    kw = dict(epsilon_eff=2.0, cfs_alpha=0.05)
+    uE, uH = ref.updates_with_cpml(fc.cpml_layers(ref.cpml_params, dt, faces, thickness, **kw), dt, dxes, eps,
+                                   dtype=pdt)
+    gE, gH = fdtd.updates_with_cpml(fc.cpml_layers(fdtd.cpml_params, dt, faces, thickness, **kw), dt, dxes, eps,
+                                    dtype=pdt)
+    et, ht = torch.from_numpy(e.copy()).cuda(), torch.from_numpy(h.copy()).cuda()
+    epst = torch.from_numpy(eps).cuda()
+    mut = None if mu is None else torch.from_numpy(mu).cuda()
+    for _ in range(6):
+        if mu is None:
+            uH(e, h)
+            gH(et, ht)
+        else:
+            uH(e, h, mu)
+            gH(et, ht, mut)
+        uE(e, h, eps)
+        gE(et, ht, epst)
+    scale = max(numpy.abs(e).max(), numpy.abs(h).max())
+    numpy.testing.assert_allclose(et.cpu().numpy(), e, rtol=tol, atol=tol * scale)
+    numpy.testing.assert_allclose(ht.cpu().numpy(), h, rtol=tol, atol=tol * scale)
+
+
+@pytest.mark.parametrize('axis,comp', [(0, 1), (1, 2), (2, 0)])
+def test_cpml_absorbs_a_pulse_on_gpu(axis, comp):
+    from meanas_amd import fdtd
+    upd = lambda p, dt, dxes, eps: fdtd.updates_with_cpml(p, dt, dxes, eps)
+    final, top = fc.pulse_energy(upd, axis, comp, make=fdtd.cpml_params,
+                                 to_dev=lambda a: torch.from_numpy(a).cuda(), to_host=lambda t: t.cpu().numpy())
+    assert final < 1e-8 and top < 1.05
+
+
+def test_cpml_numpy_fields_and_defaults():
+    """numpy fields are written back in place; mu defaults to (1, 1, 1); no layers and
+    equal E/H widths give the plain updates."""
+    from meanas_amd import fdtd
+    e, h, eps, _, dxes = _cpml_case((6, 5, 4), [], 3, 7, numpy.float64, False)
+    dxes = [dxes[1], dxes[1]]
+    gE, gH = fdtd.updates_with_cpml([[None, None]] * 3, 0.2, dxes, eps, dtype=numpy.float64)
+    e1, h1 = e.copy(), h.copy()
+    assert gH(e1, h1) is None and gE(e1, h1, eps) is None
+    e2, h2 = e.copy(), h.copy()
+    ref.maxwell_h(0.2, dxes)(e2, h2)
+    ref.maxwell_e(0.2, dxes)(e2, h2, eps)
+    numpy.testing.assert_allclose(e1, e2, rtol=1e-12, atol=1e-12)
+    numpy.testing.assert_allclose(h1, h2, rtol=1e-12, atol=1e-12)
+
+
+def test_cpml_rejects_what_has_no_kernel_form():
+    import ctypes
+    from meanas_amd import _lib, fdtd
+    shape = (8, 8, 8)
+    eps = numpy.ones((3, *shape))
+    lay = fdtd.cpml_params(0, -1, 0.3, thickness=3)
+    bad = dict(lay, region=(slice(2, 5), slice(None), slice(None)))
+    with pytest.raises(ValueError):
+        fdtd.updates_with_cpml([[bad, None], [None, None], [None, None]], 0.3, None, eps)
+    with pytest.raises(ValueError):   # a polarity -1 layer in the +1 slot
+        fdtd.updates_with_cpml([[None, lay], [None, None], [None, None]], 0.3, None, eps)
+    # the C ABI refuses layers out of (axis, polarity) order
+    gE, _ = fdtd.updates_with_cpml([[lay, fdtd.cpml_params(0, 1, 0.3, thickness=3)], [None, None], [None, None]],
+                                   0.3, None, eps)
+    tab = (_lib.MxCpmlLayer * 2)()
+    ptrs = [a.data_ptr() for a in gE.psi_state[:4]]
+    tab[0] = _lib.MxCpmlLayer(0, 1, 3, 0, *ptrs)
+    tab[1] = _lib.MxCpmlLayer(0, -1, 3, 0, *ptrs)
+    e = torch.zeros((3, *shape), dtype=torch.float64, device='cuda')
+    grid = _lib.MxFdtdGrid(8, 8, 8, 0, (ctypes.c_void_p * 3)(), (ctypes.c_void_p * 3)())
+    rc = _lib.lib().mx_fdtd_cpml_e(ctypes.byref(grid), _lib.MX_REAL64, _lib.MX_REAL32, 0.3, e.data_ptr(), e.data_ptr(),
+                                   None, 1.0, tab, 2, None)
+    assert rc != 0 and b'order' in _lib.lib().mx_last_error()

@@ -42,6 +42,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.MxStats) == 16 + 16
     assert ctypes.sizeof(_lib.MxGrid) == 16 + 6 * 8
     assert ctypes.sizeof(_lib.MxFdtdGrid) == 16 + 6 * 8
+    assert ctypes.sizeof(_lib.MxCpmlLayer) == 16 + 4 * 8
 
 
 def test_work_bytes_is_a_pure_host_query():

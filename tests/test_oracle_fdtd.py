@@ -23,3 +23,93 @@ def test_functional_curls_equal_sparse():
                                   rtol=1e-12, atol=1e-12)
     numpy.testing.assert_allclose(ref.vec(ref.f_curl_back(dxes[1])(f)), ref.curl_back(dxes[1]) @ ref.vec(f),
                                   rtol=1e-12, atol=1e-12)
+
+
+# ---- CPML: the reference ships no test of meanas/fdtd/pml.py, so the oracle is pinned
+# by a reduction to the plain Yee updates, by the stretched-grid equivalence of the
+# kappa profile, by the orientation of the profiles, and by pulse absorption.
+def _random_state(shape, seed):
+    rng = numpy.random.default_rng(seed)
+    e, h = rng.standard_normal((3, *shape)), rng.standard_normal((3, *shape))
+    eps = 1 + rng.random((3, *shape))
+    dx = [0.7 + 0.6 * rng.random(n) for n in shape]
+    return e, h, eps, dx
+
+
+def test_cpml_without_layers_is_plain_yee():
+    """No layers, equal E and H widths: the plain maxwell_h / maxwell_e updates."""
+    shape = (7, 6, 5)
+    e, h, eps, dx = _random_state(shape, 1)
+    dxes = [dx, dx]
+    uE, uH = ref.updates_with_cpml([[None, None]] * 3, 0.3, dxes, eps, dtype=numpy.float64)
+    e2, h2 = e.copy(), h.copy()
+    for _ in range(3):
+        uH(e, h)
+        uE(e, h, eps)
+        ref.maxwell_h(0.3, dxes)(e2, h2)
+        ref.maxwell_e(0.3, dxes)(e2, h2, eps)
+    numpy.testing.assert_allclose(e, e2, rtol=1e-12, atol=1e-12)
+    numpy.testing.assert_allclose(h, h2, rtol=1e-12, atol=1e-12)
+
+
+def test_cpml_profile_orientation_and_depth():
+    """Polarity -1 mirrors polarity +1 with E and H swapped; at full depth sigma and
+    kappa reach their maxima (p0 = exp(-sigma_max/kappa_max dt), p1 = p0 - 1)."""
+    t, dt, m, lnr = 7, 0.4, 3.5, -1.6
+    kw = dict(thickness=t, epsilon_eff=2.0, mu_eff=1.5, cfs_alpha=0.05)
+    lo, hi = ref.cpml_params(1, -1, dt, **kw), ref.cpml_params(1, 1, dt, **kw)
+    for a, b in zip(lo['param_e'], hi['param_h']):
+        numpy.testing.assert_allclose(a.ravel(), b.ravel()[::-1], rtol=1e-15)
+    for a, b in zip(lo['param_h'], hi['param_e']):
+        numpy.testing.assert_allclose(a.ravel(), b.ravel()[::-1], rtol=1e-15)
+    assert lo['param_e'][0].shape == (1, t, 1)
+    assert lo['region'][1] == slice(None, t) and hi['region'][1] == slice(-t, None)
+    smax, kmax = -lnr / 2 * (m + 1), numpy.sqrt(3.0)
+    p0 = numpy.exp(-smax / kmax * dt)
+    numpy.testing.assert_allclose([p.ravel()[-1] for p in hi['param_h']], [p0, p0 - 1, 1 / kmax], rtol=1e-14)
+    with pytest.raises(Exception):
+        ref.cpml_params(0, 1, dt, thickness=2)
+    with pytest.raises(Exception):
+        ref.cpml_params(3, 1, dt)
+
+
+def test_cpml_kappa_only_is_a_stretched_grid():
+    """sigma = 0 (ln_R = 0) with alpha > 0 everywhere: psi stays zero and the layers only
+    divide the derivatives along their axis by kappa, i.e. the plain updates on widths
+    multiplied by kappa inside each layer (E: kappa at the E points on the backward
+    differences; H: at the H points on the forward ones; a cell in two layers gets both).
+    The short z axis has overlapping layers."""
+    shape = (12, 10, 5)
+    e, h, eps, dx = _random_state(shape, 2)
+    dt = 0.25
+    kw = dict(ln_R_per_layer=0.0, epsilon_eff=4.0, cfs_alpha=0.3, ma=0)
+    layers = fc.cpml_layers(ref.cpml_params, dt, fc.ALL_FACES, 3, **kw)
+    uE, uH = ref.updates_with_cpml(layers, dt, [dx, dx], eps, dtype=numpy.float64)
+    dxe = [d.copy() for d in dx]
+    dxh = [d.copy() for d in dx]
+    for axis in range(3):
+        for lay in layers[axis]:
+            reg = lay['region'][axis]
+            dxe[axis][reg] /= lay['param_e'][2].ravel()
+            dxh[axis][reg] /= lay['param_h'][2].ravel()
+    assert any(not numpy.allclose(a, b) for a, b in zip(dxe, dx))
+    e2, h2 = e.copy(), h.copy()
+    for _ in range(4):
+        uH(e, h)
+        uE(e, h, eps)
+        ref.maxwell_h(dt, [dxh, None])(e2, h2)
+        ref.maxwell_e(dt, [None, dxe])(e2, h2, eps)
+    numpy.testing.assert_allclose(e, e2, rtol=1e-11, atol=1e-12)
+    numpy.testing.assert_allclose(h, h2, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize('axis,comp', [(a, c) for a in range(3) for c in range(3) if c != a])
+def test_cpml_absorbs_a_pulse(axis, comp):
+    """A pulse on a line terminated by layers at both ends leaves through them (a wrong
+    sign, a swapped u/v pairing or a missing decay factor keeps or amplifies it); with no
+    layers the periodic line keeps its energy."""
+    upd = lambda p, dt, dxes, eps: ref.updates_with_cpml(p, dt, dxes, eps)
+    final, top = fc.pulse_energy(upd, axis, comp, make=ref.cpml_params)
+    assert final < 1e-8 and top < 1.05
+    final0, _ = fc.pulse_energy(upd, axis, comp, make=ref.cpml_params, layers=False, steps=200)
+    assert final0 > 0.9
