@@ -182,10 +182,28 @@ def fdtd_workload(args):
         cells = n ** 3
         b_h = 3 * elem * 3 * cells                 # e read, h read, h written
         b_e = 4 * 3 * elem * cells                 # h, e, eps read, e written
+        # the same step with 10-cell CPML layers on all six faces (psi in float32)
+        layers = [[fdtd.cpml_params(a, p, 0.3, thickness=10) for p in (-1, 1)] for a in range(3)]
+        uE, uH = fdtd.updates_with_cpml(layers, 0.3, dxes, eps)
+        for _ in range(2):
+            uH(e, h)
+            uE(e, h, eps)
+        cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        csteps = args.steps * 5
+        cev[0].record(stream)
+        for _ in range(csteps):
+            uH(e, h)
+            uE(e, h, eps)
+        cev[1].record(stream)
+        torch.cuda.synchronize()
+        us_cpml = cev[0].elapsed_time(cev[1]) * 1e3 / csteps
         out[name] = dict(step_us=round(us_step, 2), mcell_updates_per_s=round(cells / us_step, 1),
                          h_update=dict(us=round(us_h, 2), bytes=b_h, gbs=round(b_h / us_h / 1e3, 1)),
-                         e_update=dict(us=round(us_e, 2), bytes=b_e, gbs=round(b_e / us_e / 1e3, 1)))
+                         e_update=dict(us=round(us_e, 2), bytes=b_e, gbs=round(b_e / us_e / 1e3, 1)),
+                         cpml_step=dict(us=round(us_cpml, 2), mcell_updates_per_s=round(cells / us_cpml, 1),
+                                        layers='10 cells on all six faces, psi float32'))
         log(f'fdtd {name}: {out[name]}')
+        del uE, uH
         del e, h, eps
     line = dict(metric='FDTD cell updates/s (Yee leapfrog step: H then E update)', value=out['f64']['mcell_updates_per_s'] * 1e6,
                 unit='cell-updates/s', n_gpus=1, dtype='f64 (f32 beside it)', data='synthetic',

@@ -66,8 +66,9 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 }
 
 // Occupancy target (the second launch-bounds argument is the minimum number of waves
-// per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs about 190 VGPRs,
-// and holding it to 3 waves (168) spills, so it runs 2.
+// per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs 214 VGPRs, and
+// holding it to 3 waves (168) spills 117 of them to scratch, so it runs 2.
+// MX_APPLY_WAVES64 overrides the complex128 target for tuning builds.
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
 #endif
