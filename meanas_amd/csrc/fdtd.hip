@@ -10,6 +10,7 @@
 // re-read from the caches (k+-1 is the next lane; j+-1 and i+-1 rows and planes are in
 // L2 from the neighbouring threads).  HBM-bound: per step the H update reads e and h and
 // writes h, the E update reads h and e (and eps) and writes e.
+#include <cstdlib>
 #include <string>
 
 #include "common.hpp"
@@ -73,6 +74,216 @@ __global__ __launch_bounds__(kFB) void k_fdtd_e(RGrid g, T dt, T* __restrict__ e
     for (int q = 0; q < 3; ++q) {
       const T m = eps ? eps[q * nc + cell] : eps_s;
       e[q * nc + cell] += dt * c[q] / m;
+    }
+  }
+}
+
+// ---- fused leapfrog step (mx_fdtd_run) ----------------------------------------------
+// One pass per step: H' = H - dt curl_f(E) / mu, then E' = E + dt curl_b(H') / eps, read
+// from (e0, h0) and written to (e1, h1).  The two buffer pairs alternate between steps,
+// so no block ever reads a half-updated field: 5 field streams per step instead of the
+// 7 of two separate updates.  A block owns a TY x 64 (y, z) tile and marches along x
+// over `slab` planes.  Per plane i:
+//   - E(i) of the tile plus a one-cell halo sits in LDS; plane i+1 comes from the
+//     registers of the threads that loaded it: each thread loads exactly the plane-i+1
+//     values its own H computations need (own cell; wave 0 the row j0-1 whose H it forms;
+//     lanes 0..TY-1 of wave TY-1 the column k0-1 whose H they form);
+//   - H'(i) is formed on the tile, the -y row and the -z column (the halo cells
+//     redundantly with the neighbouring blocks, by the same expressions) into LDS;
+//   - E'(i) takes H'(i) from LDS and H'(i-1) of its own cell from registers.
+// The expressions are those of k_fdtd_h and k_fdtd_e, so a run equals the step-by-step
+// updates bit for bit.  Partial tiles wrap their row and column indices (periodic grid),
+// so every load is in bounds and every neighbour read is the periodic one.
+constexpr int kStepTY = 4;
+constexpr int kStepSlab = 16;
+
+__device__ __forceinline__ int wrap(int v, int n) {
+  const int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kStepTY) void k_fdtd_step(RGrid g, T dt, const T* __restrict__ e0,
+                                                            const T* __restrict__ h0, T* __restrict__ e1,
+                                                            T* __restrict__ h1, const T* __restrict__ eps, T eps_s,
+                                                            const T* __restrict__ mu, T mu_s, int slab) {
+  constexpr int TY = kStepTY, TZ = 64, LY = TY + 2, LZ = TZ + 2;
+  static_assert(2 * TY + 2 <= 64, "the z-halo loads take 2 TY + 2 lanes of one wave");
+  __shared__ T sE[3][LY][LZ];
+  __shared__ T sH[3][TY + 1][TZ + 1];
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nc = nx * ny * nz, plane = ny * nz;
+  const int ntj = (ny + TY - 1) / TY, ntk = (nz + TZ - 1) / TZ, tiles = ntj * ntk;
+  const int nwork = ((nx + slab - 1) / slab) * tiles;
+  // workgroups are dealt round-robin to the 8 XCDs: XCD x takes work items
+  // [x * per, (x + 1) * per), neighbouring tiles of one slab, so halos hit its own L2
+  const int per = (nwork + 7) / 8;
+  const int w = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (w >= nwork) return;
+  const int slab_id = w / tiles, tile = w - slab_id * tiles;
+  const int j0 = (tile / ntk) * TY, k0 = (tile % ntk) * TZ;
+  const int ib = slab_id * slab, ie = min(ib + slab, nx);
+  const int tz = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int j = j0 + ty, k = k0 + tz;
+  const bool valid = j < ny && k < nz;
+  const int jw = wrap(j, ny), kw = wrap(k, nz);
+
+  // load sites: own cell; y-halo row (wave 0: row j0-1, wave TY-1: row j0+TY); z-halo
+  // cells (wave TY-1: lanes 0..TY column k0-1 at LDS rows 1..TY+1, lanes TY+1..2TY+1
+  // column k0+TZ at LDS rows 0..TY)
+  const int own_off = jw * nz + kw;
+  const bool has_row = ty == 0 || ty == TY - 1;
+  const int row_j = wrap(ty == 0 ? j0 - 1 : j0 + TY, ny);
+  const int row_off = row_j * nz + kw;
+  const int row_r = ty == 0 ? 0 : TY + 1;
+  const bool has_col = ty == TY - 1 && tz < 2 * TY + 2;
+  const int col_r = tz < TY + 1 ? tz + 1 : tz - (TY + 1);
+  const int col_q = tz < TY + 1 ? 0 : TZ + 1;
+  const int col_j = wrap(j0 - 1 + col_r, ny), col_k = wrap(col_q ? k0 + TZ : k0 - 1, nz);
+  const int col_off = col_j * nz + col_k;
+  const bool row_h = ty == 0;                    // forms H' of row j0-1 (x, z components)
+  const bool col_h = ty == TY - 1 && tz < TY;    // forms H' of column k0-1, row j0+tz (x, y)
+
+  // widths of the cells this thread forms H' / E' at
+  const T dye = (T)width(g.dxe[1], jw), dze = (T)width(g.dxe[2], kw);
+  const T dyh = (T)width(g.dxh[1], jw), dzh = (T)width(g.dxh[2], kw);
+  const T dye_r = (T)width(g.dxe[1], row_j);
+  const T dye_c = (T)width(g.dxe[1], col_j), dze_c = (T)width(g.dxe[2], col_k);
+
+  // What step i needs besides LDS: E(i+1) at the load sites, H(i) and mu(i) where this
+  // thread forms H', eps(i) at its own cell.
+  struct Bundle {
+    T eo[3], er[3], ec[3];   // E(i+1): own, row, column site
+    T ho[3], hr[2], hc[2];   // H(i): own; row site (x, z); column site (x, y)
+    T mo[3], mr[2], mc[2];   // mu(i) at the same cells
+    T ep[3];                 // eps(i), own cell
+  };
+  auto mat = [&](const T* m, T ms, int idx) -> T { return m ? m[idx] : ms; };
+  auto fetch = [&](Bundle& B, int i) {   // plane i+1 of E, plane i of H, mu and eps
+    const int pn = wrap(i + 1, nx) * plane, pi = i * plane;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B.eo[c] = e0[c * nc + pn + own_off];
+    if (has_row) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B.er[c] = e0[c * nc + pn + row_off];
+    }
+    if (has_col) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) B.ec[c] = e0[c * nc + pn + col_off];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      B.ho[c] = h0[c * nc + pi + own_off];
+      B.mo[c] = mat(mu, mu_s, c * nc + pi + own_off);
+      B.ep[c] = mat(eps, eps_s, c * nc + pi + own_off);
+    }
+    if (row_h) {
+      B.hr[0] = h0[pi + row_off];
+      B.hr[1] = h0[2 * nc + pi + row_off];
+      B.mr[0] = mat(mu, mu_s, pi + row_off);
+      B.mr[1] = mat(mu, mu_s, 2 * nc + pi + row_off);
+    }
+    if (col_h) {
+      B.hc[0] = h0[pi + col_off];
+      B.hc[1] = h0[nc + pi + col_off];
+      B.mc[0] = mat(mu, mu_s, pi + col_off);
+      B.mc[1] = mat(mu, mu_s, nc + pi + col_off);
+    }
+  };
+  auto put_e = [&](int i) {   // E(i) at the load sites into LDS (plane-i values of the bundle)
+    const int p = i * plane;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sE[c][ty + 1][tz + 1] = e0[c * nc + p + own_off];
+    if (has_row) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sE[c][row_r][tz + 1] = e0[c * nc + p + row_off];
+    }
+    if (has_col) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sE[c][col_r][col_q] = e0[c * nc + p + col_off];
+    }
+  };
+  auto commit = [&](const Bundle& B) {   // E(i+1) from the bundle into LDS
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sE[c][ty + 1][tz + 1] = B.eo[c];
+    if (has_row) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sE[c][row_r][tz + 1] = B.er[c];
+    }
+    if (has_col) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sE[c][col_r][col_q] = B.ec[c];
+    }
+  };
+
+  Bundle B;
+  T hpy, hpz;   // H'(i-1) y, z at the own cell
+  const int r = ty + 1, q = tz + 1;
+  // ---- prologue: H'(ib-1) at the own cell (y, z), plane ib into LDS
+  {
+    const int im1 = wrap(ib - 1, nx);
+    put_e(im1);
+    fetch(B, im1);   // E(ib), H(ib-1), mu(ib-1)
+    __syncthreads();
+    const T dx = (T)width(g.dxe[0], im1);
+    const T cy = (sE[0][r][q + 1] - sE[0][r][q]) / dze - (B.eo[2] - sE[2][r][q]) / dx;
+    const T cz = (B.eo[1] - sE[1][r][q]) / dx - (sE[0][r + 1][q] - sE[0][r][q]) / dye;
+    hpy = B.ho[1] - dt * cy / B.mo[1];
+    hpz = B.ho[2] - dt * cz / B.mo[2];
+    __syncthreads();
+    commit(B);       // plane ib
+    fetch(B, ib);    // E(ib+1), H(ib), mu(ib), eps(ib)
+  }
+
+#pragma unroll 1
+  for (int i = ib; i < ie; ++i) {
+    __syncthreads();   // E(i) is in LDS; the previous step's reads of sH are done
+    const T dxe_i = (T)width(g.dxe[0], i), dxh_i = (T)width(g.dxh[0], i);
+    const T e_own[3] = {sE[0][r][q], sE[1][r][q], sE[2][r][q]};
+    const T epsv[3] = {B.ep[0], B.ep[1], B.ep[2]};
+    // H'(i): own cell
+    {
+      const T cx = (sE[2][r + 1][q] - e_own[2]) / dye - (sE[1][r][q + 1] - e_own[1]) / dze;
+      const T cy = (sE[0][r][q + 1] - e_own[0]) / dze - (B.eo[2] - e_own[2]) / dxe_i;
+      const T cz = (B.eo[1] - e_own[1]) / dxe_i - (sE[0][r + 1][q] - e_own[0]) / dye;
+      const T hn[3] = {B.ho[0] - dt * cx / B.mo[0], B.ho[1] - dt * cy / B.mo[1], B.ho[2] - dt * cz / B.mo[2]};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sH[c][r][q] = hn[c];
+        if (valid) h1[c * nc + i * plane + own_off] = hn[c];
+      }
+    }
+    if (row_h) {   // row j0-1 (LDS row 0): x and z
+      const T cx = (sE[2][1][q] - sE[2][0][q]) / dye_r - (sE[1][0][q + 1] - sE[1][0][q]) / dze;
+      const T cz = (B.er[1] - sE[1][0][q]) / dxe_i - (sE[0][1][q] - sE[0][0][q]) / dye_r;
+      sH[0][0][q] = B.hr[0] - dt * cx / B.mr[0];
+      sH[2][0][q] = B.hr[1] - dt * cz / B.mr[1];
+    }
+    if (col_h) {   // column k0-1 (LDS column 0), row j0+tz: x and y
+      const int rr = tz + 1;
+      const T cx = (sE[2][rr + 1][0] - sE[2][rr][0]) / dye_c - (sE[1][rr][1] - sE[1][rr][0]) / dze_c;
+      const T cy = (sE[0][rr][1] - sE[0][rr][0]) / dze_c - (B.ec[2] - sE[2][rr][0]) / dxe_i;
+      sH[0][rr][0] = B.hc[0] - dt * cx / B.mc[0];
+      sH[1][rr][0] = B.hc[1] - dt * cy / B.mc[1];
+    }
+    __syncthreads();   // H'(i) is in LDS; every read of E(i) is done
+    if (i + 1 < ie) {
+      commit(B);          // E(i+1) replaces E(i)
+      fetch(B, i + 1);    // E(i+2), H(i+1), mu(i+1), eps(i+1)
+    }
+    // E'(i): own cell
+    {
+      const T hx = sH[0][r][q], hy = sH[1][r][q], hz = sH[2][r][q];
+      const T cx = (hz - sH[2][r - 1][q]) / dyh - (hy - sH[1][r][q - 1]) / dzh;
+      const T cy = (hx - sH[0][r][q - 1]) / dzh - (hz - hpz) / dxh_i;
+      const T cz = (hy - hpy) / dxh_i - (hx - sH[0][r - 1][q]) / dyh;
+      const T c3[3] = {cx, cy, cz};
+      if (valid) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) e1[c * nc + i * plane + own_off] = e_own[c] + dt * c3[c] / epsv[c];
+      }
+      hpy = hy;
+      hpz = hz;
     }
   }
 }
@@ -241,6 +452,37 @@ hipError_t launch_cpml(const RGrid& v, int prec, int psi_prec, double dt, const 
   return hipGetLastError();
 }
 
+// `steps` fused steps, (e, h) -> (e2, h2) -> (e, h) -> ...
+hipError_t launch_steps(const RGrid& v, int prec, double dt, void* e, void* h, void* e2, void* h2, const void* eps,
+                        double eps_s, const void* mu, double mu_s, int steps, hipStream_t s) {
+  static const int forced = [] {   // MEANAS_AMD_FDTD_SLAB overrides the slab depth (tuning runs)
+    const char* env = getenv("MEANAS_AMD_FDTD_SLAB");
+    return env ? atoi(env) : 0;
+  }();
+  const int want = forced > 0 ? forced : kStepSlab;
+  const int slab = v.nx < want ? v.nx : want;
+  const long long tiles = (long long)((v.ny + kStepTY - 1) / kStepTY) * ((v.nz + 63) / 64);
+  const long long nwork = (long long)((v.nx + slab - 1) / slab) * tiles;
+  const unsigned nb = (unsigned)(((nwork + 7) / 8) * 8);
+  for (int t = 0; t < steps; ++t) {
+    void* ei = (t & 1) ? e2 : e;
+    void* hi = (t & 1) ? h2 : h;
+    void* eo = (t & 1) ? e : e2;
+    void* ho = (t & 1) ? h : h2;
+    if (prec == MX_REAL64)
+      hipLaunchKernelGGL(k_fdtd_step<double>, dim3(nb), dim3(64, kStepTY), 0, s, v, dt, (const double*)ei,
+                         (const double*)hi, (double*)eo, (double*)ho, (const double*)eps, eps_s, (const double*)mu,
+                         mu_s, slab);
+    else
+      hipLaunchKernelGGL(k_fdtd_step<float>, dim3(nb), dim3(64, kStepTY), 0, s, v, (float)dt, (const float*)ei,
+                         (const float*)hi, (float*)eo, (float*)ho, (const float*)eps, (float)eps_s, (const float*)mu,
+                         (float)mu_s, slab);
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
+}
+
 int check_args(const mx_fdtd_grid* g, int prec, const void* a, const void* b, RGrid* v, const char* who) {
   std::string why;
   if (!rgrid(g, v, &why)) return set_error(std::string(who) + ": " + why);
@@ -275,13 +517,22 @@ int mx_fdtd_run(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, void* h
   mx::RGrid v;
   if (mx::check_args(g, prec, e, h, &v, "mx_fdtd_run")) return -1;
   if (steps < 0) return mx::set_error("mx_fdtd_run: steps < 0");
+  if (steps == 0) return 0;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  for (int t = 0; t < steps; ++t) {
-    hipError_t err = mx::launch_h(v, prec, dt, e, h, mu, mu_scalar, s);
-    if (err == hipSuccess) err = mx::launch_e(v, prec, dt, e, h, eps, eps_scalar, s);
-    if (err != hipSuccess) return mx::set_error(std::string("mx_fdtd_run: ") + hipGetErrorString(err));
+  // fused steps alternate between (e, h) and a second pair, allocated in stream order
+  const size_t bytes = (size_t)3 * v.nx * v.ny * v.nz * (prec == MX_REAL64 ? 8 : 4);
+  void* e2 = nullptr;
+  void* h2 = nullptr;
+  hipError_t err = hipMallocAsync(&e2, bytes, s);
+  if (err == hipSuccess) err = hipMallocAsync(&h2, bytes, s);
+  if (err == hipSuccess) err = mx::launch_steps(v, prec, dt, e, h, e2, h2, eps, eps_scalar, mu, mu_scalar, steps, s);
+  if (err == hipSuccess && (steps & 1)) {   // the last step wrote the second pair
+    err = hipMemcpyAsync(e, e2, bytes, hipMemcpyDeviceToDevice, s);
+    if (err == hipSuccess) err = hipMemcpyAsync(h, h2, bytes, hipMemcpyDeviceToDevice, s);
   }
-  return 0;
+  if (e2) (void)hipFreeAsync(e2, s);
+  if (h2) (void)hipFreeAsync(h2, s);
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_run: ") + hipGetErrorString(err));
 }
 
 int mx_fdtd_cpml_e(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, void* e, const void* h,

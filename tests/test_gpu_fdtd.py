@@ -37,23 +37,29 @@ def _fields(shape, seed, dtype):
     return e, h, eps, mu, dxes
 
 
-@pytest.mark.parametrize('shape', [(9, 10, 70), (1, 12, 5), (7, 1, 1)])
-def test_run_matches_steps_and_oracle(shape):
+@pytest.mark.parametrize('shape', [(9, 10, 70), (1, 12, 5), (7, 1, 1), (40, 7, 100), (33, 5, 64), (17, 3, 130)])
+@pytest.mark.parametrize('steps', [5, 6])
+@pytest.mark.parametrize('dtype', [numpy.float64, numpy.float32])
+def test_run_matches_steps_and_oracle(shape, steps, dtype):
+    """The fused runner (one pass per step over a ping-pong pair; slabs, partial y and z
+    tiles, odd step counts) equals the separate H and E updates bit for bit."""
     from meanas_amd import fdtd
-    e, h, eps, mu, dxes = _fields(shape, 1, numpy.float64)
+    e, h, eps, mu, dxes = _fields(shape, 1, dtype)
     dt = 0.2
     et = torch.from_numpy(e.copy()).cuda()
     ht = torch.from_numpy(h.copy()).cuda()
-    fdtd.run(et, ht, eps, dt, dxes, steps=6, mu=mu)
+    fdtd.run(et, ht, eps, dt, dxes, steps=steps, mu=mu)
     e2, h2 = e.copy(), h.copy()
     me, mh = fdtd.maxwell_e(dt, dxes), fdtd.maxwell_h(dt, dxes)
-    for _ in range(6):
+    for _ in range(steps):
         mh(e2, h2, mu)
         me(e2, h2, eps)
-    numpy.testing.assert_array_equal(et.cpu().numpy(), e2)   # same kernels, same order
+    numpy.testing.assert_array_equal(et.cpu().numpy(), e2)   # same expressions, same order
     numpy.testing.assert_array_equal(ht.cpu().numpy(), h2)
+    if dtype == numpy.float32:
+        return
     e3, h3 = e.copy(), h.copy()
-    for _ in range(6):
+    for _ in range(steps):
         ref.maxwell_h(dt, dxes)(e3, h3, mu)
         ref.maxwell_e(dt, dxes)(e3, h3, eps)
     numpy.testing.assert_allclose(e2, e3, rtol=1e-11, atol=1e-12)
@@ -67,6 +73,13 @@ def test_unit_widths_scalar_materials_and_float32():
     dt = 0.25
     e64, h64 = e.copy(), h.copy()
     fdtd.run(e64, h64, 2.0, dt, None, steps=5, mu=1.5)          # numpy in, updated in place
+    e1, h1 = e.copy(), h.copy()                                 # the separate updates, bit for bit
+    me, mh = fdtd.maxwell_e(dt), fdtd.maxwell_h(dt)
+    for _ in range(5):
+        mh(e1, h1, 1.5)
+        me(e1, h1, 2.0)
+    numpy.testing.assert_array_equal(e64, e1)
+    numpy.testing.assert_array_equal(h64, h1)
     eo, ho = e.copy(), h.copy()
     for _ in range(5):
         ref.maxwell_h(dt)(eo, ho, 1.5)
