@@ -210,7 +210,10 @@ int mx_fdtd_h(const mx_fdtd_grid* g, int32_t prec, double dt, const void* e, voi
 int mx_fdtd_e(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, const void* h, const void* eps,
               double eps_scalar, void* stream);
 /* `steps` leapfrog steps: maxwell_h then maxwell_e, as the reference's update loop
- * (meanas/test/test_fdtd.py:215-233) runs them, without sources. */
+ * (meanas/test/test_fdtd.py:215-233) runs them, without sources.  Each step is one fused
+ * pass that gives the same bits as mx_fdtd_h followed by mx_fdtd_e; the passes alternate
+ * between (e, h) and a second pair of fields that the call allocates and frees in stream
+ * order (hipMallocAsync, 2 x 3N values), and the result ends in (e, h). */
 int mx_fdtd_run(const mx_fdtd_grid* g, int32_t prec, double dt, void* e, void* h, const void* eps,
                 double eps_scalar, const void* mu, double mu_scalar, int32_t steps, void* stream);
 
