@@ -69,6 +69,13 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 // per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs 214 VGPRs, and
 // holding it to 3 waves (168) spills 117 of them to scratch, so it runs 2.
 // MX_APPLY_WAVES64 overrides the complex128 target for tuning builds.
+// Waves 0 and TY-1 also form the halo row / column of H, so they finish the H phase
+// last and the others wait at the barrier; while they do it they run at wave priority
+// MX_APPLY_HALO_PRIO (s_setprio; 0 = off).  Measured: the in-solve c128 apply 56.1 ->
+// 53.9 us at 128^3; levels 1-3 alike.
+#ifndef MX_APPLY_HALO_PRIO
+#define MX_APPLY_HALO_PRIO 1
+#endif
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
 #endif
@@ -295,11 +302,17 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     };
     using T_ = std::integral_constant<bool, true>;
     using F_ = std::integral_constant<bool, false>;
+#if MX_APPLY_HALO_PRIO
+    if (ty == 0 || ty == TY - 1) __builtin_amdgcn_s_setprio(MX_APPLY_HALO_PRIO);   // the waves with halo H work
+#endif
     if (rx) {
       if (ry) h_phase(T_{}, T_{}); else h_phase(T_{}, F_{});
     } else {
       if (ry) h_phase(F_{}, T_{}); else h_phase(F_{}, F_{});
     }
+#if MX_APPLY_HALO_PRIO
+    __builtin_amdgcn_s_setprio(0);
+#endif
     __syncthreads();   // H(i) is in LDS; every read of plane i is done
     if (i + 1 < ie) {
       commit(Rn);                                         // plane i+1 replaces plane i
