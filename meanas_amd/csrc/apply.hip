@@ -69,12 +69,20 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 // per SIMD): complex64 fits 4 waves (<= 128 VGPRs); complex128 needs 214 VGPRs, and
 // holding it to 3 waves (168) spills 117 of them to scratch, so it runs 2.
 // MX_APPLY_WAVES64 overrides the complex128 target for tuning builds.
-// Waves 0 and TY-1 also form the halo row / column of H, so they finish the H phase
-// last and the others wait at the barrier; while they do it they run at wave priority
-// MX_APPLY_HALO_PRIO (s_setprio; 0 = off).  Measured: the in-solve c128 apply 56.1 ->
-// 53.9 us at 128^3; levels 1-3 alike.
+//
+// Halo work.  Besides its own row, a tile forms H on the halo row j0-1 (Hx, Hz) and the
+// halo column k0-1 (Hx, Hy).  Hz of the row and Hy of the column need plane i+1 from
+// the registers of the waves that loaded those sites (0 and TY-1); the two Hx parts read
+// only LDS.  MX_APPLY_SPLIT_HALO = 1 gives the Hx parts to waves 1 and TY-2, so no wave
+// forms more than one extra component and the others wait less at the barrier.
+// MX_APPLY_HALO_PRIO > 0 instead raises the priority (s_setprio) of waves 0 and TY-1
+// during the H phase.  Measured in the 128^3 solve (c128 apply): neither 56.1 us,
+// priority 53.9-55.7 us, split 54.7 us; split plus priority no better than the split.
+#ifndef MX_APPLY_SPLIT_HALO
+#define MX_APPLY_SPLIT_HALO 1
+#endif
 #ifndef MX_APPLY_HALO_PRIO
-#define MX_APPLY_HALO_PRIO 1
+#define MX_APPLY_HALO_PRIO 0
 #endif
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
@@ -150,7 +158,13 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   const int col_c = tz < TY + 1 ? 0 : TZ + 1;
   const int col_off = has_col ? wrapi(j0 - 1 + col_r, ny) * nz + wrapi(col_c ? k0 + TZ : k0 - 1, nz) : 0;
   const int col_lds = col_r * LZ + col_c;
-  const bool col_halo_h = (ty == TY - 1) && (tz < TY);   // halo H at column k0-1, LDS row tz+1
+  const bool col_halo_h = (ty == TY - 1) && (tz < TY);   // halo Hy at column k0-1, LDS row tz+1
+  // waves forming the LDS-only halo components (Hx of the row and of the column)
+#if MX_APPLY_SPLIT_HALO
+  constexpr int kRowHx = 1, kColHx = TY - 2;
+#else
+  constexpr int kRowHx = 0, kColHx = TY - 1;
+#endif
   const int nc = (int)ncell;
 
   // One plane's worth of this thread's loads (own column, y-halo row, z-halo cell).
@@ -271,8 +285,8 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     // Real-coefficient fast paths: x coefficients are uniform over the plane, y
     // coefficients over a wave (one wave is one y row); z varies per lane and stays complex.
     const bool rx = is_real(cxi.fa) && is_real(cxi.fb) && is_real(fa_xm);
-    const bool ry = __builtin_amdgcn_readfirstlane(
-        is_real(cyj.fa) && is_real(cyj.fb) && is_real(fa_ym) && (ty != 0 || (is_real(cyh.fa) && is_real(cyh.fb))));
+    const bool ry = __builtin_amdgcn_readfirstlane(is_real(cyj.fa) && is_real(cyj.fb) && is_real(fa_ym) &&
+                                                   ((ty != 0 && ty != kRowHx) || (is_real(cyh.fa) && is_real(cyh.fb))));
     auto h_phase = [&](auto RXt, auto RYt) {
       constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
       const cplx<T> hx = dif<RY>(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif<false>(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
@@ -281,22 +295,35 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
       sH[0][r][q] = hscale(hx, 0, mown, cell);
       sH[1][r][q] = hscale(hy, 1, mown, cell);
       sH[2][r][q] = hscale(hz, 2, mown, cell);
-      if (ty == 0) {   // halo row j0-1 (LDS row 0): Hx and Hz are consumed
+      // Halo row j0-1 (LDS row 0) needs Hx and Hz, halo column k0-1 (LDS column 0, rows
+      // 1..TY) Hx and Hy.  Hz of the row and Hy of the column use plane i+1, which is in
+      // the registers of the waves that loaded those sites (0 and TY-1); the Hx parts
+      // read only LDS and go to waves 1 and TY-2, so no wave forms more than one
+      // extra component.
+      if (ty == kRowHx) {   // halo row: Hx
         const uint8_t mh = sM[0][q];
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
         const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
-        const cplx<T> gz = dif<RX>(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[0][0][q] = hscale(gx, 0, mh, hcell);
+      }
+      if (ty == 0) {   // halo row: Hz
+        const uint8_t mh = sM[0][q];
+        const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
+        const cplx<T> gz = dif<RX>(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[2][0][q] = hscale(gz, 2, mh, hcell);
       }
-      if (col_halo_h) {   // halo column k0-1 (LDS col 0), rows 1..TY: Hx and Hy
+      if (ty == kColHx && tz < TY) {   // halo column: Hx
         const int rr = tz + 1;
-        const int jj = wrapi(j0 + tz, ny);
         const uint8_t mh = sM[rr][0];
-        const int hcell = i * (int)plane + jj * nz + wrapi(k0 - 1, nz);
+        const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
-        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
+      }
+      if (col_halo_h) {   // halo column: Hy
+        const int rr = tz + 1;
+        const uint8_t mh = sM[rr][0];
+        const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
+        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[1][rr][0] = hscale(gy, 1, mh, hcell);
       }
     };
