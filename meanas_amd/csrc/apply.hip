@@ -87,10 +87,16 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
 #endif
-template <typename T> constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : MX_APPLY_WAVES64;
+// The simple-material kernels (SM) need 174 VGPRs in complex128: 3 waves would spill 8
+// and measured slower (46.8 against 44.6 us in the 128^3 solve), so they also run 2.
+#ifndef MX_APPLY_WAVES64_SM
+#define MX_APPLY_WAVES64_SM 2
+#endif
+template <typename T, bool SM>
+constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : (SM ? MX_APPLY_WAVES64_SM : MX_APPLY_WAVES64);
 
-template <typename T, int TY, int EPI, typename RT, int PD>
-__global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpView<T>* __restrict__ ops,
+template <typename T, int TY, int EPI, typename RT, int PD, bool SM>
+__global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const OpView<T>* __restrict__ ops,
                                                                  const SysDesc* __restrict__ systems,
                                                                  const BlockMap* __restrict__ bmap,
                                                                  int in_slot, int out_slot,
@@ -132,8 +138,12 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   const bool valid = (j < ny) && (k < nz);
   const int jw = wrapi(j, ny), kw = wrapi(k, nz);
 
-  const bool has_mask = (op.flags & OPF_MASK) != 0;
-  const bool has_mu = (op.flags & OPF_MU) != 0;
+  // SM (simple materials): no PEC/PMC mask, no mu, real isotropic eps, known at compile
+  // time, so that code and its registers drop out of the kernel
+  const bool has_mask = !SM && (op.flags & OPF_MASK) != 0;
+  const bool has_mu = !SM && (op.flags & OPF_MU) != 0;
+  const bool eps_cplx = !SM && (op.flags & OPF_EPS_COMPLEX) != 0;
+  const bool eps_aniso = !SM && (op.flags & OPF_EPS_ANISO) != 0;
   const AxisCoef<T>* cx = op.coef;
   const AxisCoef<T>* cy = op.coef + nx;
   const AxisCoef<T>* cz = op.coef + nx + ny;
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     cplx<T>* base = &sE[0][0][0];
 #pragma unroll
     for (int c = 0; c < 3; ++c) base[c * LY * LZ + lds] = pec(v, m, c);
-    (&sM[0][0])[lds] = m;
+    if (has_mask) (&sM[0][0])[lds] = m;
   };
   auto commit = [&](const Regs& R) {
     put(own_lds, R.o, R.mo);
@@ -232,7 +242,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
   auto load_eps = [&](int cell) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      epsn[c] = load_mat<T>(op.eps, op.flags & OPF_EPS_COMPLEX, op.flags & OPF_EPS_ANISO, c, cell, nc);
+      epsn[c] = load_mat<T>(op.eps, eps_cplx, eps_aniso, c, cell, nc);
   };
 
   // ---- prologue: H(ib-1) at the own column (y, z components); plane ib into LDS
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
     const cplx<T> n1 = pec(R.o, R.mo, 1), n2 = pec(R.o, R.mo, 2);
     cplx<T> hy = (czk.fa * e_(0, r, q + 1) - czk.fb * e_(0, r, q)) - (cxm.fa * n2 - cxm.fb * e_(2, r, q));
     cplx<T> hz = (cxm.fa * n1 - cxm.fb * e_(1, r, q)) - (cyj.fa * e_(0, r + 1, q) - cyj.fb * e_(0, r, q));
-    const uint8_t m = sM[r][q];
+    const uint8_t m = has_mask ? sM[r][q] : 0;
     const int cell = im1 * (int)plane + own_off;
     hpy = hscale(hy, 1, m, cell);
     hpz = hscale(hz, 2, m, cell);
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const cplx<T> w2 = op.w2;
-  const bool diag_real = !(op.flags & OPF_EPS_COMPLEX) && w2.im == T(0);
+  const bool diag_real = !eps_cplx && w2.im == T(0);
   T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
 
     const int r = ty + 1, q = tz + 1;
     cplx<T> ec[3] = {e_(0, r, q), e_(1, r, q), e_(2, r, q)};
-    const uint8_t mown = sM[r][q];
+    const uint8_t mown = has_mask ? sM[r][q] : 0;
     // Real-coefficient fast paths: x coefficients are uniform over the plane, y
     // coefficients over a wave (one wave is one y row); z varies per lane and stays complex.
     const bool rx = is_real(cxi.fa) && is_real(cxi.fb) && is_real(fa_xm);
@@ -301,27 +311,27 @@ __global__ __launch_bounds__(64 * TY, kWavesPerSimd<T>) void k_apply(const OpVie
       // read only LDS and go to waves 1 and TY-2, so no wave forms more than one
       // extra component.
       if (ty == kRowHx) {   // halo row: Hx
-        const uint8_t mh = sM[0][q];
+        const uint8_t mh = has_mask ? sM[0][q] : 0;
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
         const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
         sH[0][0][q] = hscale(gx, 0, mh, hcell);
       }
       if (ty == 0) {   // halo row: Hz
-        const uint8_t mh = sM[0][q];
+        const uint8_t mh = has_mask ? sM[0][q] : 0;
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
         const cplx<T> gz = dif<RX>(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[2][0][q] = hscale(gz, 2, mh, hcell);
       }
       if (ty == kColHx && tz < TY) {   // halo column: Hx
         const int rr = tz + 1;
-        const uint8_t mh = sM[rr][0];
+        const uint8_t mh = has_mask ? sM[rr][0] : 0;
         const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
       }
       if (col_halo_h) {   // halo column: Hy
         const int rr = tz + 1;
-        const uint8_t mh = sM[rr][0];
+        const uint8_t mh = has_mask ? sM[rr][0] : 0;
         const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
         const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[1][rr][0] = hscale(gy, 1, mh, hcell);
@@ -449,50 +459,55 @@ static int prefetch_depth(size_t elem) {
 }
 
 template <typename T, int EPI, typename RT>
-static hipError_t launch_apply_ty(int ty, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
+static hipError_t launch_apply_ty(int ty, bool simple, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
                                   const void* in_ptr = nullptr, void* out_ptr = nullptr) {
   const int pd = prefetch_depth(sizeof(T));
-#define MX_APPLY_CASE(TY_, PD_)                                                                              \
-  case TY_ * 10 + PD_:                                                                                     \
-    hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys, bmap, \
-                       in_slot, out_slot, skip, need, in_ptr, out_ptr);                                    \
+  // simple-material kernels exist for prefetch depth 1 only (depth 2 is a tuning override)
+  const int sm = simple && pd == 1;
+#define MX_APPLY_CASE(TY_, PD_, SM_)                                                                           \
+  case (TY_ * 10 + PD_) * 2 + SM_:                                                                            \
+    hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_, SM_ != 0>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys, \
+                       bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr);                                    \
     break;
-  switch (ty * 10 + pd) {
-    MX_APPLY_CASE(2, 1)
-    MX_APPLY_CASE(4, 1)
-    MX_APPLY_CASE(8, 1)
-    MX_APPLY_CASE(2, 2)
-    MX_APPLY_CASE(4, 2)
-    MX_APPLY_CASE(8, 2)
+  switch ((ty * 10 + pd) * 2 + sm) {
+    MX_APPLY_CASE(2, 1, 0)
+    MX_APPLY_CASE(4, 1, 0)
+    MX_APPLY_CASE(8, 1, 0)
+    MX_APPLY_CASE(2, 2, 0)
+    MX_APPLY_CASE(4, 2, 0)
+    MX_APPLY_CASE(8, 2, 0)
+    MX_APPLY_CASE(2, 1, 1)
+    MX_APPLY_CASE(4, 1, 1)
+    MX_APPLY_CASE(8, 1, 1)
     default: return hipErrorInvalidValue;
   }
 #undef MX_APPLY_CASE
   return hipGetLastError();
 }
 
-hipError_t launch_apply_plain_f32(int ty, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
                                   const void* x, void* y, hipStream_t s) {
-  return launch_apply_ty<float, EPI_PLAIN, float>(ty, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+  return launch_apply_ty<float, EPI_PLAIN, float>(ty, simple, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
 }
-hipError_t launch_apply_plain_f64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
                                   const void* x, void* y, hipStream_t s) {
-  return launch_apply_ty<double, EPI_PLAIN, double>(ty, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+  return launch_apply_ty<double, EPI_PLAIN, double>(ty, simple, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
 }
-hipError_t launch_apply_cocg_f32(int ty, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<float, EPI_COCG, float>(ty, ops, sys, bmap, nb, VS_P, VS_Q,
+hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<float, EPI_COCG, float>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
                                                  SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
-hipError_t launch_apply_cocg_f64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_COCG, double>(ty, ops, sys, bmap, nb, VS_P, VS_Q,
+hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_COCG, double>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
                                                    SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
 // residual r = b - A x in FP64; r stored as float (mixed mode) or double (FP64 mode)
-hipError_t launch_apply_resid_r32(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_RESID, float>(ty, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, float>(ty, simple, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
 }
-hipError_t launch_apply_resid_r64(int ty, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_RESID, double>(ty, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, double>(ty, simple, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
 }
 
 }  // namespace mx

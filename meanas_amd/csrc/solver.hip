@@ -17,12 +17,12 @@
 
 namespace mx {
 // apply.hip
-hipError_t launch_apply_plain_f32(int ty, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
-hipError_t launch_apply_plain_f64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
-hipError_t launch_apply_cocg_f32(int ty, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_cocg_f64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_resid_r32(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_resid_r64(int ty, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 // vector.hip
 hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
@@ -104,6 +104,13 @@ constexpr int kRefreshIters = 2000;   // forced FP64 residual replacement cadenc
 // least once per stall window.
 int refresh_iters(int stall_iters) {
   return stall_iters > 0 ? std::min(stall_iters, kRefreshIters) : kRefreshIters;
+}
+
+// Simple materials: no PEC/PMC mask, no mu, real isotropic eps.  The apply has a
+// kernel specialised for them (that code and its registers drop out).
+bool op_simple(const mx_operator& op) {
+  const OpView<double> v = view_of<double>(op);
+  return !(v.flags & (OPF_MASK | OPF_MU | OPF_EPS_COMPLEX | OPF_EPS_ANISO));
 }
 
 // One finishing launch per fused iteration (default); MEANAS_AMD_MERGED_FIN=0 launches
@@ -203,6 +210,7 @@ struct Batch {
   int graph_chunk = 0;
   bool classic_launch = true;   // enqueue the classic-form direction update every iteration
   bool merged_fin = false;      // every system stays fused: one finishing launch per iteration
+  bool simple = false;          // every operator has simple materials (no mask, no mu, real isotropic eps)
 
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -284,7 +292,9 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 
   std::vector<OpView<float>> v32(nops);
   std::vector<OpView<double>> v64(nops);
+  B.simple = true;
   for (int i = 0; i < nops; ++i) {
+    B.simple = B.simple && op_simple(ops[i]);
     v64[i] = view_of<double>(ops[i]);
     if (o.mode == MX_MIXED) v32[i] = view_of<float>(ops[i]);
     else std::memset(&v32[i], 0, sizeof(v32[i]));
@@ -306,18 +316,18 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 int enqueue_iteration(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
     if (B.merged_fin) {   // the previous update is finished together with this apply
-      MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
       MX_TRY(launch_fin(FIN_APPLY_MERGED, B.d_sys, B.d_syslist, B.nsl, s));
       MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
       return 0;
     }
     if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   }
@@ -330,10 +340,10 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
 int enqueue_replacement(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_accum_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
-    MX_TRY(launch_apply_resid_r32(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_resid_r32(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
   } else {
     MX_TRY(launch_accum_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
-    MX_TRY(launch_apply_resid_r64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_resid_r64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
   }
   MX_TRY(launch_fin(FIN_RESID, B.d_sys, B.d_syslist, B.nsl, s));
   MX_TRY(launch_save_best(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
@@ -421,6 +431,7 @@ size_t mx_work_bytes(const mx_operator* op, int32_t mode) {
 struct mx_plan {
   int mode, ty, nb;
   char* d_mem;
+  bool simple;
 };
 
 mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, int32_t slab) {
@@ -442,7 +453,7 @@ mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, 
   const size_t s_op = align_up(std::max(sizeof(OpView<float>), sizeof(OpView<double>)));
   const size_t s_sys = align_up(sizeof(SysDesc));
   const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
-  mx_plan* P = new (std::nothrow) mx_plan{mode, ty, d.nblk_apply, nullptr};
+  mx_plan* P = new (std::nothrow) mx_plan{mode, ty, d.nblk_apply, nullptr, op_simple(*op)};
   if (!P) { fail("out of host memory"); return nullptr; }
   hipError_t e = hipMalloc(&P->d_mem, s_op + s_sys + s_bm);
   if (e != hipSuccess) { fail(std::string("hipMalloc: ") + hipGetErrorString(e)); delete P; return nullptr; }
@@ -467,9 +478,9 @@ int mx_plan_apply(mx_plan* P, const void* x, void* y, void* stream) {
   const BlockMap* bm = reinterpret_cast<const BlockMap*>(P->d_mem + s_op + s_sys);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (P->mode == MX_MIXED)
-    MX_TRY(launch_apply_plain_f32(P->ty, reinterpret_cast<const OpView<float>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+    MX_TRY(launch_apply_plain_f32(P->ty, P->simple, reinterpret_cast<const OpView<float>*>(P->d_mem), sys, bm, P->nb, x, y, s));
   else
-    MX_TRY(launch_apply_plain_f64(P->ty, reinterpret_cast<const OpView<double>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+    MX_TRY(launch_apply_plain_f64(P->ty, P->simple, reinterpret_cast<const OpView<double>*>(P->d_mem), sys, bm, P->nb, x, y, s));
   return 0;
 }
 
@@ -539,7 +550,7 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   if (e == hipSuccess) e = hipMemcpy(mem + s_op + s_sys + s_st + s_part, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
   const int zero_idx = 0;
   if (e == hipSuccess) e = hipMemcpy(d_one, &zero_idx, sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_apply_resid_r64(ty, reinterpret_cast<const OpView<double>*>(mem), reinterpret_cast<const SysDesc*>(mem + s_op),
+  if (e == hipSuccess) e = launch_apply_resid_r64(ty, op_simple(*op), reinterpret_cast<const OpView<double>*>(mem), reinterpret_cast<const SysDesc*>(mem + s_op),
                                                  reinterpret_cast<const BlockMap*>(mem + s_op + s_sys + s_st + s_part), d.nblk_apply, s);
   if (e == hipSuccess) e = launch_fin(FIN_RESID, reinterpret_cast<const SysDesc*>(mem + s_op), d_one, 1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s);
@@ -586,8 +597,8 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
     switch (k) {
       case 0: return fp64 ? launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s)
                           : launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
-      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
-                          : launch_apply_cocg_f32(B.ty, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
+      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
+                          : launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
       case 2: return launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s);
       case 3: return fp64 ? launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s)
                           : launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s);
@@ -838,6 +849,9 @@ Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const
   B.d_bmap_vec = const_cast<BlockMap*>(bm_vec);
   B.nb_apply = nb_apply;
   B.nb_vec = nb_vec;
+  B.simple = true;
+  for (int i = 0; i < S->capacity; ++i)
+    if (S->active[i]) B.simple = B.simple && op_simple(S->h_ops[i]);
   return B;
 }
 
