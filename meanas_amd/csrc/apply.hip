@@ -87,8 +87,9 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
 #endif
-// The simple-material kernels (SM) need 174 VGPRs in complex128: 3 waves would spill 8
-// and measured slower (46.8 against 44.6 us in the 128^3 solve), so they also run 2.
+// The simple-material kernels (SM) need 166 VGPRs in complex128, so they can hold 3
+// waves per SIMD; at the 128^3 tiling (512 blocks of 4 waves) only 2 are used either way,
+// and deeper occupancy measured no faster (slab sweep in the solve), so the bound stays 2.
 #ifndef MX_APPLY_WAVES64_SM
 #define MX_APPLY_WAVES64_SM 2
 #endif
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
   __shared__ uint8_t sM[LY][LZ];      // its PEC/PMC mask bits
   __shared__ cplx<T> sH[3][TY + 1][TZ + 1];
   __shared__ double s_red[4 * 16];
+  __shared__ FaFb<T> sCy[TY];         // y coefficients of the tile rows (the column halo's)
 
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
@@ -222,10 +224,10 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
   const cplx<T> fa_ym = coef_u(cy, wrapi(j - 1, ny)).fa, fa_zm = coef_at(cz, wrapi(k - 1, nz)).fa;
   const FaFb<T> cyh = coef_u(cy, wrapi(j0 - 1, ny));   // halo row j0-1
   const FaFb<T> czh = coef_u(cz, wrapi(k0 - 1, nz));   // halo column k0-1
-  // y coefficients of the row whose column-halo H this lane computes (lanes tz < TY of
-  // wave TY-1).  Loaded once here: a global load consumed inside the plane loop would
-  // make the wave wait for every plane load in flight (the memory counter is in order).
-  const FaFb<T> cyr = coef_at(cy, wrapi(j0 + (tz < TY ? tz : 0), ny));
+  // y coefficients of the rows whose column-halo H the lanes tz < TY form: each wave's
+  // own row coefficients, shared through LDS (the prologue's barrier orders it), so the
+  // plane loop reads them from LDS and no thread keeps them in registers
+  if (tz == 0) sCy[ty] = cyj;
 
   // H multiplier at (cell, component): pmc keep * 1/mu
   auto hscale = [&](cplx<T> h, int c, uint8_t m, int cell) -> cplx<T> {
@@ -326,6 +328,7 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
         const int rr = tz + 1;
         const uint8_t mh = has_mask ? sM[rr][0] : 0;
         const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
+        const FaFb<T> cyr = sCy[tz];
         const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
       }
