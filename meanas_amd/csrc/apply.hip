@@ -44,14 +44,13 @@ __device__ __forceinline__ cplx<T> load_mat(const T* base, bool is_complex, bool
   return cplx<T>{gload1<T>(base, idx), T(0)};
 }
 
-// fa x1 - fb x0.  REAL: both coefficients are real (the axis is outside the PML here),
-// which halves the multiplications.
-template <bool REAL, typename T>
+// fa x1 - fb x0 with complex coefficients.  (Real-coefficient variants for axes outside
+// the PML, selected per plane and per wave, made the kernel 1.46x larger and measured 3%
+// slower in the 128^3 solve, so there is one form.)
+template <typename T>
 __device__ __forceinline__ cplx<T> dif(const cplx<T>& fa, const cplx<T>& x1, const cplx<T>& fb, const cplx<T>& x0) {
-  if constexpr (REAL) return cplx<T>{fa.re * x1.re - fb.re * x0.re, fa.re * x1.im - fb.re * x0.im};
-  else return fa * x1 - fb * x0;
+  return fa * x1 - fb * x0;
 }
-template <typename T> __device__ __forceinline__ bool is_real(const cplx<T>& a) { return a.im == T(0); }
 
 // forward-difference coefficients (fa, fb) of entry n of a per-axis table; coef_u for
 // an n that is uniform over the wave (scalar loads, SGPRs)
@@ -294,16 +293,13 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
     const int r = ty + 1, q = tz + 1;
     cplx<T> ec[3] = {e_(0, r, q), e_(1, r, q), e_(2, r, q)};
     const uint8_t mown = has_mask ? sM[r][q] : 0;
-    // Real-coefficient fast paths: x coefficients are uniform over the plane, y
-    // coefficients over a wave (one wave is one y row); z varies per lane and stays complex.
-    const bool rx = is_real(cxi.fa) && is_real(cxi.fb) && is_real(fa_xm);
-    const bool ry = __builtin_amdgcn_readfirstlane(is_real(cyj.fa) && is_real(cyj.fb) && is_real(fa_ym) &&
-                                                   ((ty != 0 && ty != kRowHx) || (is_real(cyh.fa) && is_real(cyh.fb))));
-    auto h_phase = [&](auto RXt, auto RYt) {
-      constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
-      const cplx<T> hx = dif<RY>(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif<false>(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
-      const cplx<T> hy = dif<false>(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif<RX>(cxi.fa, pec(Rn.o, Rn.mo, 2), cxi.fb, ec[2]);
-      const cplx<T> hz = dif<RX>(cxi.fa, pec(Rn.o, Rn.mo, 1), cxi.fb, ec[1]) - dif<RY>(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
+#if MX_APPLY_HALO_PRIO
+    if (ty == 0 || ty == TY - 1) __builtin_amdgcn_s_setprio(MX_APPLY_HALO_PRIO);   // the waves with halo H work
+#endif
+    {   // H phase
+      const cplx<T> hx = dif(cyj.fa, e_(2, r + 1, q), cyj.fb, ec[2]) - dif(czk.fa, e_(1, r, q + 1), czk.fb, ec[1]);
+      const cplx<T> hy = dif(czk.fa, e_(0, r, q + 1), czk.fb, ec[0]) - dif(cxi.fa, pec(Rn.o, Rn.mo, 2), cxi.fb, ec[2]);
+      const cplx<T> hz = dif(cxi.fa, pec(Rn.o, Rn.mo, 1), cxi.fb, ec[1]) - dif(cyj.fa, e_(0, r + 1, q), cyj.fb, ec[0]);
       sH[0][r][q] = hscale(hx, 0, mown, cell);
       sH[1][r][q] = hscale(hy, 1, mown, cell);
       sH[2][r][q] = hscale(hz, 2, mown, cell);
@@ -315,13 +311,13 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
       if (ty == kRowHx) {   // halo row: Hx
         const uint8_t mh = has_mask ? sM[0][q] : 0;
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
-        const cplx<T> gx = dif<RY>(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif<false>(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
+        const cplx<T> gx = dif(cyh.fa, e_(2, 1, q), cyh.fb, e_(2, 0, q)) - dif(czk.fa, e_(1, 0, q + 1), czk.fb, e_(1, 0, q));
         sH[0][0][q] = hscale(gx, 0, mh, hcell);
       }
       if (ty == 0) {   // halo row: Hz
         const uint8_t mh = has_mask ? sM[0][q] : 0;
         const int hcell = i * (int)plane + wrapi(j0 - 1, ny) * nz + kw;
-        const cplx<T> gz = dif<RX>(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif<RY>(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
+        const cplx<T> gz = dif(cxi.fa, pec(Rn.r, Rn.mr, 1), cxi.fb, e_(1, 0, q)) - dif(cyh.fa, e_(0, 1, q), cyh.fb, e_(0, 0, q));
         sH[2][0][q] = hscale(gz, 2, mh, hcell);
       }
       if (ty == kColHx && tz < TY) {   // halo column: Hx
@@ -329,26 +325,16 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
         const uint8_t mh = has_mask ? sM[rr][0] : 0;
         const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
         const FaFb<T> cyr = sCy[tz];
-        const cplx<T> gx = dif<false>(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif<false>(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
+        const cplx<T> gx = dif(cyr.fa, e_(2, rr + 1, 0), cyr.fb, e_(2, rr, 0)) - dif(czh.fa, e_(1, rr, 1), czh.fb, e_(1, rr, 0));
         sH[0][rr][0] = hscale(gx, 0, mh, hcell);
       }
       if (col_halo_h) {   // halo column: Hy
         const int rr = tz + 1;
         const uint8_t mh = has_mask ? sM[rr][0] : 0;
         const int hcell = i * (int)plane + wrapi(j0 + tz, ny) * nz + wrapi(k0 - 1, nz);
-        const cplx<T> gy = dif<false>(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif<RX>(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
+        const cplx<T> gy = dif(czh.fa, e_(0, rr, 1), czh.fb, e_(0, rr, 0)) - dif(cxi.fa, pec(Rn.c, Rn.mc, 2), cxi.fb, e_(2, rr, 0));
         sH[1][rr][0] = hscale(gy, 1, mh, hcell);
       }
-    };
-    using T_ = std::integral_constant<bool, true>;
-    using F_ = std::integral_constant<bool, false>;
-#if MX_APPLY_HALO_PRIO
-    if (ty == 0 || ty == TY - 1) __builtin_amdgcn_s_setprio(MX_APPLY_HALO_PRIO);   // the waves with halo H work
-#endif
-    if (rx) {
-      if (ry) h_phase(T_{}, T_{}); else h_phase(T_{}, F_{});
-    } else {
-      if (ry) h_phase(F_{}, T_{}); else h_phase(F_{}, F_{});
     }
 #if MX_APPLY_HALO_PRIO
     __builtin_amdgcn_s_setprio(0);
@@ -361,18 +347,9 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
     }
 
     const cplx<T> hx0 = sH[0][r][q], hy0 = sH[1][r][q], hz0 = sH[2][r][q];
-    cplx<T> yx, yy, yz;
-    auto y_phase = [&](auto RXt, auto RYt) {
-      constexpr bool RX = decltype(RXt)::value, RY = decltype(RYt)::value;
-      yx = dif<RY>(cyj.fb, hz0, fa_ym, sH[2][r - 1][q]) - dif<false>(czk.fb, hy0, fa_zm, sH[1][r][q - 1]);
-      yy = dif<false>(czk.fb, hx0, fa_zm, sH[0][r][q - 1]) - dif<RX>(cxi.fb, hz0, fa_xm, hpz);
-      yz = dif<RX>(cxi.fb, hy0, fa_xm, hpy) - dif<RY>(cyj.fb, hx0, fa_ym, sH[0][r - 1][q]);
-    };
-    if (rx) {
-      if (ry) y_phase(T_{}, T_{}); else y_phase(T_{}, F_{});
-    } else {
-      if (ry) y_phase(F_{}, T_{}); else y_phase(F_{}, F_{});
-    }
+    const cplx<T> yx = dif(cyj.fb, hz0, fa_ym, sH[2][r - 1][q]) - dif(czk.fb, hy0, fa_zm, sH[1][r][q - 1]);
+    const cplx<T> yy = dif(czk.fb, hx0, fa_zm, sH[0][r][q - 1]) - dif(cxi.fb, hz0, fa_xm, hpz);
+    const cplx<T> yz = dif(cxi.fb, hy0, fa_xm, hpy) - dif(cyj.fb, hx0, fa_ym, sH[0][r - 1][q]);
     hpy = hy0;
     hpz = hz0;
     cplx<T> out[3] = {yx, yy, yz};
