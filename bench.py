@@ -296,7 +296,7 @@ def main():
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
         print(f'bench.py measures one GPU only: --gpus {args.gpus} / WORLD_SIZE {world} is not supported '
-              '(this build has no multi-GPU path; see DESIGN.md for how the solve would shard)')
+              '(there is no multi-GPU path: independent systems would go one per GPU, one process each)')
         sys.exit(2)
 
     import torch
