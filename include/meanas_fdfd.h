@@ -244,6 +244,17 @@ int mx_fdtd_cpml_e(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double
 int mx_fdtd_cpml_h(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double dt, const void* e, void* h,
                    const void* mu, double mu_scalar, const mx_cpml_layer* layers, int32_t nlayers, void* stream);
 
+/* Conducting boundary of one face (fdtd/boundaries.py:12-78, conducting_boundary): the
+ * in-place update that its en/hn (polarity -1) or ep/hp (polarity +1) make to e
+ * (field = MX_FIELD_E) or h (MX_FIELD_H), with the evident tuple indexing (the
+ * reference indexes with a list of slices, which numpy >= 1.23 rejects).  Polarity -1
+ * rewrites plane 0 from plane 1; +1 rewrites planes n-1 and n-2 from n-2 and n-3, so the
+ * axis needs at least 2 or 3 cells.  g's widths are not read. */
+#define MX_FIELD_E 0
+#define MX_FIELD_H 1
+int mx_fdtd_conducting_boundary(const mx_fdtd_grid* g, int32_t prec, int32_t direction, int32_t polarity,
+                                int32_t field, void* f, void* stream);
+
 /* Kernel timing of solver iterations (measurement; no counterpart in the reference).
  * Sets up `sys` as mx_solve would (r = b - A x, p = r) with every stopping rule off, runs
  * `reps` iterations on `stream` exactly as a solve launches them, with a HIP event

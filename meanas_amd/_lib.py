@@ -27,6 +27,7 @@ MX_NONFINITE = 3
 MX_STALLED = 4
 MX_FORM_AUTO, MX_FORM_FUSED, MX_FORM_CLASSIC = 0, 1, 2
 MX_REAL32, MX_REAL64 = 0, 1
+MX_FIELD_E, MX_FIELD_H = 0, 1
 FORMS = {'auto': MX_FORM_AUTO, 'fused': MX_FORM_FUSED, 'classic': MX_FORM_CLASSIC}
 STATUS_NAMES = {0: 'converged', 1: 'maxiter', 2: 'breakdown', 3: 'nonfinite', 4: 'stalled'}
 
@@ -137,6 +138,8 @@ PROTOTYPES = {
     'mx_fdtd_cpml_h': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
                                       c_void_p, c_void_p, c_void_p, ctypes.c_double, ctypes.POINTER(MxCpmlLayer),
                                       ctypes.c_int32, c_void_p]),
+    'mx_fdtd_conducting_boundary': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
     'mx_time_kernels': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), ctypes.POINTER(MxOptions),
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, c_void_p]),
     'mx_session_create': (c_void_p, [ctypes.c_int32, ctypes.POINTER(MxOptions)]),

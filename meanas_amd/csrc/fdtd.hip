@@ -379,6 +379,61 @@ __global__ __launch_bounds__(kFB) void k_cpml(RGrid g, T dt, T* __restrict__ e, 
   }
 }
 
+// ---- conducting boundary (fdtd/boundaries.py:12-78) ---------------------------------
+// One face's planes along `dir` are rewritten in place, one thread per (u, v) position
+// of the face (u < v the other two axes).  A thread reads its column before it writes
+// it and touches no other column, so threads need no ordering.
+//   polarity -1, planes 0 and 1:
+//     E: f_d[0] = 0, f_u[0] = f_u[1], f_v[0] = f_v[1]
+//     H: f_d[0] = f_d[1], f_u[0] = f_v[0] = 0
+//   polarity +1, planes L = n - 1, L - 1, L - 2:
+//     E: f_d[L] = -f_d[L-2], f_d[L-1] = 0, f_u[L] = f_u[L-1], f_v[L] = f_v[L-1]
+//     H: f_d[L] = f_d[L-1], f_u[L] = -f_u[L-2], f_u[L-1] = 0, and the same for v
+// These are the image values of a mirror plane at 1/2 (polarity -1) or n - 3/2 (+1)
+// across which tangential E and normal H are even, normal E and tangential H odd.
+template <typename T>
+__global__ __launch_bounds__(kFB) void k_conducting(int nx, int ny, int nz, int dir, int pol, int is_h,
+                                                    T* __restrict__ f) {
+  const int dims[3] = {nx, ny, nz};
+  const long long strides[3] = {(long long)ny * nz, nz, 1};
+  const int u = dir == 0 ? 1 : 0, v = dir == 2 ? 1 : 2;
+  const long long nc = (long long)nx * ny * nz;
+  const int nplane = dims[u] * dims[v];
+  const int last = dims[dir] - 1;
+  for (int t = blockIdx.x * kFB + threadIdx.x; t < nplane; t += gridDim.x * kFB) {
+    const long long base = (t / dims[v]) * strides[u] + (t % dims[v]) * strides[v];
+    auto at = [&](int comp, int plane) -> T& { return f[comp * nc + base + plane * strides[dir]]; };
+    if (pol < 0) {
+      if (!is_h) {
+        const T fu = at(u, 1), fv = at(v, 1);
+        at(dir, 0) = T(0);
+        at(u, 0) = fu;
+        at(v, 0) = fv;
+      } else {
+        const T fd = at(dir, 1);
+        at(dir, 0) = fd;
+        at(u, 0) = T(0);
+        at(v, 0) = T(0);
+      }
+    } else {
+      if (!is_h) {
+        const T fd = at(dir, last - 2), fu = at(u, last - 1), fv = at(v, last - 1);
+        at(dir, last) = -fd;
+        at(dir, last - 1) = T(0);
+        at(u, last) = fu;
+        at(v, last) = fv;
+      } else {
+        const T fd = at(dir, last - 1), fu = at(u, last - 2), fv = at(v, last - 2);
+        at(dir, last) = fd;
+        at(u, last) = -fu;
+        at(u, last - 1) = T(0);
+        at(v, last) = -fv;
+        at(v, last - 1) = T(0);
+      }
+    }
+  }
+}
+
 bool rgrid(const mx_fdtd_grid* g, RGrid* v, std::string* why) {
   if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) { *why = "bad grid"; return false; }
   if (3LL * g->nx * g->ny * g->nz >= (1LL << 31)) { *why = "grid too large (3N must be < 2^31)"; return false; }
@@ -415,6 +470,20 @@ hipError_t launch_e(const RGrid& v, int prec, double dt, void* e, const void* h,
   else
     hipLaunchKernelGGL(k_fdtd_e<float>, dim3(blocks(v)), dim3(kFB), 0, s, v, (float)dt, (float*)e, (const float*)h,
                        (const float*)eps, (float)eps_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_conducting(const RGrid& v, int prec, int dir, int pol, int is_h, void* f, hipStream_t s) {
+  const int dims[3] = {v.nx, v.ny, v.nz};
+  const long long nplane = (long long)dims[dir == 0 ? 1 : 0] * dims[dir == 2 ? 1 : 2];
+  const long long nb = (nplane + kFB - 1) / kFB;
+  const unsigned grid = (unsigned)(nb > 16384 ? 16384 : nb);
+  if (prec == MX_REAL64)
+    hipLaunchKernelGGL(k_conducting<double>, dim3(grid), dim3(kFB), 0, s, v.nx, v.ny, v.nz, dir, pol, is_h,
+                       (double*)f);
+  else
+    hipLaunchKernelGGL(k_conducting<float>, dim3(grid), dim3(kFB), 0, s, v.nx, v.ny, v.nz, dir, pol, is_h,
+                       (float*)f);
   return hipGetLastError();
 }
 
@@ -565,6 +634,22 @@ int mx_fdtd_cpml_h(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double
   const hipError_t err = mx::launch_cpml<false>(v, prec, psi_prec, dt, e, h, mu, mu_scalar, tab,
                                                 reinterpret_cast<hipStream_t>(stream));
   return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_cpml_h: ") + hipGetErrorString(err));
+}
+
+int mx_fdtd_conducting_boundary(const mx_fdtd_grid* g, int32_t prec, int32_t direction, int32_t polarity,
+                                int32_t field, void* f, void* stream) {
+  mx::RGrid v;
+  if (mx::check_args(g, prec, f, f, &v, "mx_fdtd_conducting_boundary")) return -1;
+  if (direction < 0 || direction > 2) return mx::set_error("mx_fdtd_conducting_boundary: direction must be 0, 1 or 2");
+  if (polarity != -1 && polarity != 1) return mx::set_error("mx_fdtd_conducting_boundary: polarity must be -1 or +1");
+  if (field != MX_FIELD_E && field != MX_FIELD_H)
+    return mx::set_error("mx_fdtd_conducting_boundary: field must be MX_FIELD_E or MX_FIELD_H");
+  const int dims[3] = {v.nx, v.ny, v.nz};
+  if (dims[direction] < (polarity < 0 ? 2 : 3))
+    return mx::set_error("mx_fdtd_conducting_boundary: the axis needs at least 2 cells (polarity -1) or 3 (+1)");
+  const hipError_t err = mx::launch_conducting(v, prec, direction, polarity, field == MX_FIELD_H, f,
+                                               reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_conducting_boundary: ") + hipGetErrorString(err));
 }
 
 }  // extern "C"

@@ -23,6 +23,8 @@ Restated from the reference's text (paths relative to the reference root):
                                           energy_estep, delta_energy_j, dxmul
   meanas/fdfd/operators.py:366-450        e_tfsf_source / e_boundary_source
   meanas/fdfd/waveguide_3d.py:113-156,206-252  compute_source / expand_e
+  meanas/fdtd/pml.py:23-230               cpml_params / updates_with_cpml
+  meanas/fdtd/boundaries.py:12-78         conducting_boundary (tuple indexing; see there)
 Third-party arithmetic: scipy.sparse (CSR products) and scipy.sparse.linalg.qmr
 (reference pins scipy~=1.14, pyproject.toml:44; this container has 1.15.3).
 
@@ -63,6 +65,18 @@ How each function is pinned (tests/test_oracle.py):
                        pinned together by the restated one-way PML test
                        (meanas/test/test_fdfd_pml.py:14-28): the source launches a wave
                        in one direction only (reverse |E|^2 < 1e-12, forward 1 +- 3e-6).
+  cpml              -- no reference test exists (fdtd/pml.py:8 "TODO retest pmls");
+                       pinned by: no layers and equal widths give the plain Yee update;
+                       the profiles' mirror symmetry between polarities and their
+                       closed-form sigma_max, kappa_max at full depth; sigma = 0 equals
+                       the plain update on widths multiplied by kappa inside each layer
+                       (overlapping layers included); a pulse along each axis and in each
+                       polarization leaves through the layers without growth.
+  conducting_bound. -- a bounded leapfrog run with both faces of an axis equals, plane for
+                       plane, a periodic run on the doubled domain whose fields are
+                       mirror-symmetric about cells 1/2 and n - 3/2 (tangential E and
+                       normal H even, normal E and tangential H odd); a wrong plane, sign
+                       or component breaks the equality.
 Deviation, by necessity: the adjoint branch uses `.conj().T` where the reference
 writes `.H`, an attribute scipy sparse arrays do not have (solvers.py:121).
 """
@@ -659,3 +673,57 @@ def updates_with_cpml(cpml_params, dt: float, dxes, epsilon, *, dtype=numpy.floa
             h[c] -= dt / mu[c] * (d[c1, c2] - d[c2, c1] + acc[c])
 
     return update_E, update_H
+
+
+# ---------------------------------------------------------------------------
+# FDTD conducting boundary  (meanas/fdtd/boundaries.py:12-78)
+# ---------------------------------------------------------------------------
+def conducting_boundary(direction: int, polarity: int) -> tuple[Callable, Callable]:
+    """(update_e, update_h) of boundaries.py:12-78 with tuple indices.  The reference
+    indexes `e[direction][boundary_slice]` with a list of slices, which numpy >= 1.23
+    rejects; the updates are what the same assignments do with `tuple(slice_list)`."""
+    if direction not in (0, 1, 2):
+        raise Exception(f'Invalid direction: {direction}')
+    u, v = [a for a in range(3) if a != direction]
+
+    def plane(index):
+        sl = [slice(None)] * 3
+        sl[direction] = index
+        return tuple(sl)
+
+    if polarity < 0:
+        b0, s1 = plane(0), plane(1)
+
+        def en(e):
+            e[direction][b0] = 0
+            e[u][b0] = e[u][s1]
+            e[v][b0] = e[v][s1]
+            return e
+
+        def hn(h):
+            h[direction][b0] = h[direction][s1]
+            h[u][b0] = 0
+            h[v][b0] = 0
+            return h
+
+        return en, hn
+    if polarity > 0:
+        b0, s1, s2 = plane(-1), plane(-2), plane(-3)
+
+        def ep(e):
+            e[direction][b0] = -e[direction][s2]
+            e[direction][s1] = 0
+            e[u][b0] = e[u][s1]
+            e[v][b0] = e[v][s1]
+            return e
+
+        def hp(h):
+            h[direction][b0] = h[direction][s1]
+            h[u][b0] = -h[u][s2]
+            h[u][s1] = 0
+            h[v][b0] = -h[v][s2]
+            h[v][s1] = 0
+            return h
+
+        return ep, hp
+    raise Exception(f'Bad polarity: {polarity}')

@@ -196,3 +196,59 @@ def pulse_energy(updates, axis, comp, n=160, thickness=12, steps=500, dt=0.5, la
         top = max(top, float((ee ** 2).sum() + (hh ** 2).sum()) / u0)
     ee, hh = (e, h) if to_host is None else (to_host(e), to_host(h))
     return float((ee ** 2).sum() + (hh ** 2).sum()) / u0, top
+
+
+# ---- conducting boundary: the image (mirror) construction ---------------------------
+def mirror_fields(direction, n, other=(3, 4), seed=0, dtype=numpy.float64):
+    """e, h, eps on a periodic domain of 2 (n - 2) planes along `direction`, symmetric
+    about the plane at cell 1/2 (and so, by the period, about n - 3/2).  Along the
+    direction, tangential E and normal H sit at whole cells (plane i at i) and are even;
+    normal E and tangential H sit at half cells (plane i at i + 1/2) and are odd; eps
+    follows its E component with even parity."""
+    period = 2 * (n - 2)
+    others = iter(other)
+    shape = [period if a == direction else next(others) for a in range(3)]
+    rng = numpy.random.default_rng(seed)
+    e = rng.standard_normal((3, *shape))
+    h = rng.standard_normal((3, *shape))
+    eps = 1 + rng.random((3, *shape))
+    idx = numpy.arange(period)
+    image = {True: (1 - idx) % period, False: (-idx) % period}   # whole / half cells
+
+    def sym(f, c, whole, sign):
+        f[c] = (f[c] + sign * numpy.take(f[c], image[whole], axis=direction)) / 2
+
+    for c in range(3):
+        normal = c == direction
+        sym(e, c, not normal, -1 if normal else 1)
+        sym(h, c, normal, 1 if normal else -1)
+        sym(eps, c, not normal, 1)
+    return tuple(a.astype(dtype) for a in (e, h, eps))
+
+
+def mirror_mismatch(direction, n, step_pair, boundaries, steps=8, seed=0, dtype=numpy.float64, to_dev=None,
+                    to_host=None, mutate=None):
+    """Largest difference, over `steps` leapfrog steps, between a run on planes 0..n-1
+    with both conducting faces of `direction` applied after each update and a plain
+    periodic run on the doubled mirror-symmetric domain.  step_pair(e, h, eps, after_h,
+    after_e) does one step (H update, after_h(), E update, after_e());
+    boundaries(direction, polarity) -> (update_e, update_h); `mutate(e, h)`
+    plants a change after the boundaries (for checking that the comparison sees it)."""
+    to_dev = to_dev or (lambda a: a)
+    to_host = to_host or (lambda a: a)
+    e, h, eps = mirror_fields(direction, n, seed=seed, dtype=dtype)
+    take = lambda a: numpy.ascontiguousarray(numpy.take(a, range(n), axis=1 + direction))
+    eb, hb, epsb = (to_dev(take(a)) for a in (e, h, eps))
+    e, h, eps = to_dev(e), to_dev(h), to_dev(eps)
+    en, hn = boundaries(direction, -1)
+    ep, hp = boundaries(direction, 1)
+    worst = 0.0
+    for _ in range(steps):
+        step_pair(e, h, eps, lambda: None, lambda: None)
+        step_pair(eb, hb, epsb, lambda: (hn(hb), hp(hb)), lambda: (en(eb), ep(eb)))
+        if mutate is not None:
+            mutate(eb, hb)
+        for full, part in ((e, eb), (h, hb)):
+            diff = numpy.abs(take(to_host(full)) - to_host(part)).max()
+            worst = max(worst, float(diff))
+    return worst

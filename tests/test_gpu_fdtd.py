@@ -186,3 +186,61 @@ def test_cpml_rejects_what_has_no_kernel_form():
     rc = _lib.lib().mx_fdtd_cpml_e(ctypes.byref(grid), _lib.MX_REAL64, _lib.MX_REAL32, 0.3, e.data_ptr(), e.data_ptr(),
                                    None, 1.0, tab, 2, None)
     assert rc != 0 and b'order' in _lib.lib().mx_last_error()
+
+
+# ---- conducting boundary (meanas/fdtd/boundaries.py) ------------------------------
+@pytest.mark.parametrize('direction', [0, 1, 2])
+@pytest.mark.parametrize('polarity', [-1, 1])
+@pytest.mark.parametrize('dtype', [numpy.float64, numpy.float32])
+@pytest.mark.parametrize('size', ['minimal', 'regular'])
+def test_conducting_boundary_matches_oracle(direction, polarity, dtype, size):
+    """Both field updates of one face equal the oracle's bit for bit (copies, zeros and
+    negations only); the smallest axis each polarity accepts is included."""
+    from meanas_amd import fdtd
+    shape = [5, 6, 7] if size == 'regular' else [3, 4, 1]
+    if size == 'minimal':
+        shape[direction] = 2 if polarity < 0 else 3
+    rng = numpy.random.default_rng(10 + direction + 3 * polarity)
+    for kind in (0, 1):
+        f = rng.standard_normal((3, *shape)).astype(dtype)
+        want = ref.conducting_boundary(direction, polarity)[kind](f.copy())
+        t = torch.from_numpy(f.copy()).cuda()
+        got = fdtd.conducting_boundary(direction, polarity)[kind](t)
+        assert got is t
+        numpy.testing.assert_array_equal(t.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize('direction', [0, 1, 2])
+@pytest.mark.parametrize('dtype,tol', [(numpy.float64, 1e-13), (numpy.float32, 1e-5)])
+def test_conducting_boundary_is_a_mirror_on_gpu(direction, dtype, tol):
+    """GPU updates with the GPU boundaries on both faces equal a plain periodic GPU run on
+    the doubled mirror-symmetric domain."""
+    from meanas_amd import fdtd
+    mh, me = fdtd.maxwell_h(0.3), fdtd.maxwell_e(0.3)
+
+    def step(e, h, eps, after_h, after_e):
+        mh(e, h)
+        after_h()
+        me(e, h, eps)
+        after_e()
+    worst = fc.mirror_mismatch(direction, 7, step, fdtd.conducting_boundary, dtype=dtype, seed=direction,
+                               to_dev=lambda a: torch.from_numpy(a).cuda(), to_host=lambda t: t.cpu().numpy())
+    assert worst <= tol
+
+
+def test_conducting_boundary_numpy_and_errors():
+    from meanas_amd import fdtd
+    en, hn = fdtd.conducting_boundary(0, -5)        # any negative polarity is the low face
+    f = numpy.random.default_rng(0).standard_normal((3, 4, 3, 2))
+    want = ref.conducting_boundary(0, -1)[1](f.copy())
+    assert hn(f) is f                                # numpy written back in place
+    numpy.testing.assert_array_equal(f, want)
+    ep, _ = fdtd.conducting_boundary(2, 1)
+    with pytest.raises(IndexError):                  # +1 needs three cells along the axis
+        ep(torch.zeros((3, 4, 4, 2), dtype=torch.float64, device='cuda'))
+    with pytest.raises(TypeError):
+        en(torch.zeros((3, 4, 4, 2), dtype=torch.complex128, device='cuda'))
+    with pytest.raises(Exception, match='Invalid direction'):
+        fdtd.conducting_boundary(3, 1)
+    with pytest.raises(Exception, match='Bad polarity'):
+        fdtd.conducting_boundary(1, 0)

@@ -55,6 +55,25 @@ def test_work_bytes_is_a_pure_host_query():
     assert _lib.lib().mx_work_bytes(ctypes.byref(op), 7) == 0
 
 
+def test_conducting_boundary_checks_arguments_on_the_host():
+    """mx_fdtd_conducting_boundary refuses bad arguments before any device call (each
+    call below fails a check; the buffer is never handed to a kernel)."""
+    from meanas_amd import _lib
+    L = _lib.lib()
+    buf = numpy.zeros(3 * 4 * 4 * 2)
+    ptr = buf.ctypes.data
+    grid = _lib.MxFdtdGrid(4, 4, 2, 0, (ctypes.c_void_p * 3)(), (ctypes.c_void_p * 3)())
+    cases = [((_lib.MX_REAL64, 0, -1, _lib.MX_FIELD_E, None), b'null field'),
+             ((5, 0, -1, _lib.MX_FIELD_E, ptr), b'prec'),
+             ((_lib.MX_REAL64, 3, -1, _lib.MX_FIELD_E, ptr), b'direction'),
+             ((_lib.MX_REAL64, 0, 0, _lib.MX_FIELD_E, ptr), b'polarity'),
+             ((_lib.MX_REAL64, 0, 1, 2, ptr), b'field'),
+             ((_lib.MX_REAL64, 2, 1, _lib.MX_FIELD_H, ptr), b'at least')]   # nz = 2 < 3
+    for args, msg in cases:
+        assert L.mx_fdtd_conducting_boundary(ctypes.byref(grid), *args, None) != 0
+        assert msg in L.mx_last_error(), (args, L.mx_last_error())
+
+
 def test_product_scpml_matches_oracle():
     from meanas_amd.fdfd import scpml
     shape = (30, 1, 41)

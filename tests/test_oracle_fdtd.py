@@ -113,3 +113,64 @@ def test_cpml_absorbs_a_pulse(axis, comp):
     assert final < 1e-8 and top < 1.05
     final0, _ = fc.pulse_energy(upd, axis, comp, make=ref.cpml_params, layers=False, steps=200)
     assert final0 > 0.9
+
+
+# ---- conducting boundary (meanas/fdtd/boundaries.py): no reference test exists; the
+# oracle is pinned by the image construction (a bounded run equals a periodic run on the
+# doubled mirror-symmetric domain).
+def oracle_pair(dt=0.3):
+    mh, me = ref.maxwell_h(dt), ref.maxwell_e(dt)
+
+    def step(e, h, eps, after_h, after_e):
+        mh(e, h)
+        after_h()
+        me(e, h, eps)
+        after_e()
+    return step
+
+
+@pytest.mark.parametrize('direction', [0, 1, 2])
+@pytest.mark.parametrize('n', [4, 5, 9])
+def test_conducting_boundary_is_a_mirror(direction, n):
+    assert fc.mirror_mismatch(direction, n, oracle_pair(), ref.conducting_boundary, seed=10 * direction + n) == 0.0
+
+
+def _plane(d, i):
+    sl = [slice(None)] * 3
+    sl[d] = i
+    return tuple(sl)
+
+
+@pytest.mark.parametrize('mutant', ['sign', 'plane', 'component', 'h_normal'])
+def test_mirror_check_sees_planted_mistakes(mutant):
+    """Each plausible mistake in a face update, planted after the oracle's boundaries,
+    breaks the mirror equality."""
+    d, u = 1, 0
+
+    def mutate(e, h):
+        if mutant == 'sign':          # E normal on the last plane not negated
+            e[d][_plane(d, -1)] = e[d][_plane(d, -3)]
+        elif mutant == 'plane':       # E tangential copied from the wrong plane
+            e[u][_plane(d, -1)] = e[u][_plane(d, -3)]
+        elif mutant == 'component':   # tangential H zeroed on the normal component too
+            h[d][_plane(d, 0)] = 0
+        else:                         # H normal not copied at the high face
+            h[d][_plane(d, -1)] = h[d][_plane(d, -3)]
+    assert fc.mirror_mismatch(d, 6, oracle_pair(), ref.conducting_boundary, mutate=mutate) > 1e-3
+
+
+def test_reference_indexing_needs_tuples():
+    """The reference indexes e[direction] with a list of slices; numpy >= 1.23 rejects
+    that, so the oracle uses the tuple the list spells."""
+    a = numpy.zeros((3, 4, 5))
+    with pytest.raises(IndexError):
+        a[[slice(None), slice(None), 0]] = 1
+    ep, hp = ref.conducting_boundary(2, 1)
+    f = numpy.arange(3 * 2 * 3 * 4, dtype=float).reshape(3, 2, 3, 4)
+    g = ep(f.copy())
+    assert (g[2][:, :, 2] == 0).all() and (g[2][:, :, 3] == -f[2][:, :, 1]).all()
+    assert (g[0][:, :, 3] == f[0][:, :, 2]).all() and (g[:, :, :, :2] == f[:, :, :, :2]).all()
+    with pytest.raises(Exception, match='Invalid direction'):
+        ref.conducting_boundary(3, 1)
+    with pytest.raises(Exception, match='Bad polarity'):
+        ref.conducting_boundary(0, 0)
