@@ -255,6 +255,34 @@ int mx_fdtd_cpml_h(const mx_fdtd_grid* g, int32_t prec, int32_t psi_prec, double
 int mx_fdtd_conducting_boundary(const mx_fdtd_grid* g, int32_t prec, int32_t direction, int32_t polarity,
                                 int32_t field, void* f, void* stream);
 
+/* Cell energies of fdtd/energy.py:128-295, in float64 (the reference's float64 widths
+ * promote): out = sum_c(ee_c eps_c) dV_e + sum_c(hh_c mu_c) dV_h with dV the products of
+ * g's widths (dxe for E, dxh for H; unit when NULL) and ee, hh formed from the fields
+ * f[0..3] (each 3N values in the field precision) by `form`:
+ *   MX_ENERGY_DXMUL (dxmul)             f = ee, hh
+ *   MX_ENERGY_HSTEP (energy_hstep)      f = e0, h1, e2:     ee = e0 e2, hh = h1 h1
+ *   MX_ENERGY_ESTEP (energy_estep)      f = h0, e1, h2:     ee = e1 e1, hh = h0 h2
+ *   MX_ENERGY_H2E (delta_energy_h2e)    f = e0, h1, e2, h3: ee = e2 (e2 - e0) / dt, hh = h1 (h3 - h1) / dt
+ *   MX_ENERGY_E2H (delta_energy_e2h)    f = h0, e1, h2, e3: ee = e1 (e3 - e1) / dt, hh = h2 (h2 - h0) / dt
+ *   MX_ENERGY_J (delta_energy_j)        f = j0, e1:         ee = j0 e1, no H term, eps unused
+ * eps / mu: 3N values in the field precision, or NULL for eps_scalar / mu_scalar. */
+#define MX_ENERGY_DXMUL 0
+#define MX_ENERGY_HSTEP 1
+#define MX_ENERGY_ESTEP 2
+#define MX_ENERGY_H2E 3
+#define MX_ENERGY_E2H 4
+#define MX_ENERGY_J 5
+int mx_fdtd_energy(const mx_fdtd_grid* g, int32_t prec, int32_t form, double dt, const void* const* f,
+                   const void* eps, double eps_scalar, const void* mu, double mu_scalar, double* out, void* stream);
+/* Poynting vector of fdtd/energy.py:10-87 (poynting): s_x = ey(i+1) hz - ez(i+1) hy and
+ * cyclically, E and H multiplied by their widths, E taken at the next cell (periodic);
+ * computed in double and stored in the field precision (3N values). */
+int mx_fdtd_poynting(const mx_fdtd_grid* g, int32_t prec, const void* e, const void* h, void* s, void* stream);
+/* Divergence of s (fdtd/energy.py:90-125, poynting_divergence): N values
+ * sum_a (s_a - s_a at the previous cell along a), unit spacing as the reference's
+ * deriv_back() without widths; g's widths are not read. */
+int mx_fdtd_poynting_divergence(const mx_fdtd_grid* g, int32_t prec, const void* s, void* ds, void* stream);
+
 /* Kernel timing of solver iterations (measurement; no counterpart in the reference).
  * Sets up `sys` as mx_solve would (r = b - A x, p = r) with every stopping rule off, runs
  * `reps` iterations on `stream` exactly as a solve launches them, with a HIP event

@@ -28,6 +28,7 @@ MX_STALLED = 4
 MX_FORM_AUTO, MX_FORM_FUSED, MX_FORM_CLASSIC = 0, 1, 2
 MX_REAL32, MX_REAL64 = 0, 1
 MX_FIELD_E, MX_FIELD_H = 0, 1
+MX_ENERGY_DXMUL, MX_ENERGY_HSTEP, MX_ENERGY_ESTEP, MX_ENERGY_H2E, MX_ENERGY_E2H, MX_ENERGY_J = range(6)
 FORMS = {'auto': MX_FORM_AUTO, 'fused': MX_FORM_FUSED, 'classic': MX_FORM_CLASSIC}
 STATUS_NAMES = {0: 'converged', 1: 'maxiter', 2: 'breakdown', 3: 'nonfinite', 4: 'stalled'}
 
@@ -140,6 +141,13 @@ PROTOTYPES = {
                                       ctypes.c_int32, c_void_p]),
     'mx_fdtd_conducting_boundary': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32,
                                                    ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+    'mx_fdtd_energy': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.POINTER(c_void_p), c_void_p, ctypes.c_double, c_void_p, ctypes.c_double,
+                                      c_void_p, c_void_p]),
+    'mx_fdtd_poynting': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    'mx_fdtd_poynting_divergence': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, c_void_p, c_void_p,
+                                                   c_void_p]),
     'mx_time_kernels': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.POINTER(MxSystem), ctypes.POINTER(MxOptions),
                                        ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_int32, c_void_p]),
     'mx_session_create': (c_void_p, [ctypes.c_int32, ctypes.POINTER(MxOptions)]),

@@ -434,6 +434,88 @@ __global__ __launch_bounds__(kFB) void k_conducting(int nx, int ny, int nz, int 
   }
 }
 
+// ---- energy and Poynting flux (fdtd/energy.py) ------------------------------------
+// Rounding follows the reference's array expressions: the field products and the
+// component sum in the field precision (in component order, starting from the x term),
+// the widths in double (the float64 width arrays promote) multiplied one axis at a time,
+// and nothing contracted into fused multiply-adds.  `form` is one of MX_ENERGY_*.
+
+template <typename T>
+__global__ __launch_bounds__(kFB) void k_energy(RGrid g, int form, T dt, const T* __restrict__ f0,
+                                                const T* __restrict__ f1, const T* __restrict__ f2,
+                                                const T* __restrict__ f3, const T* __restrict__ eps, T eps_s,
+                                                const T* __restrict__ mu, T mu_s, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int nc = g.nx * g.ny * g.nz;
+  for (int cell = blockIdx.x * kFB + threadIdx.x; cell < nc; cell += gridDim.x * kFB) {
+    const int k = cell % g.nz, t = cell / g.nz, j = t % g.ny, i = t / g.ny;
+    T se = T(0), sh = T(0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int q = c * nc + cell;
+      T ee, hh = T(0);
+      switch (form) {
+        case MX_ENERGY_DXMUL: ee = f0[q]; hh = f1[q]; break;
+        case MX_ENERGY_HSTEP: ee = f0[q] * f2[q]; hh = f1[q] * f1[q]; break;
+        case MX_ENERGY_ESTEP: ee = f1[q] * f1[q]; hh = f0[q] * f2[q]; break;
+        case MX_ENERGY_H2E: ee = f2[q] * (f2[q] - f0[q]) / dt; hh = f1[q] * (f3[q] - f1[q]) / dt; break;
+        case MX_ENERGY_E2H: ee = f1[q] * (f3[q] - f1[q]) / dt; hh = f2[q] * (f2[q] - f0[q]) / dt; break;
+        default: ee = f0[q] * f1[q]; break;   // j0 . e1, no material
+      }
+      const T we = form == MX_ENERGY_J ? ee : ee * (eps ? eps[q] : eps_s);
+      const T wh = hh * (mu ? mu[q] : mu_s);
+      se = c == 0 ? we : se + we;
+      sh = c == 0 ? wh : sh + wh;
+    }
+    const double ue = (double)se * width(g.dxe[0], i) * width(g.dxe[1], j) * width(g.dxe[2], k);
+    out[cell] = form == MX_ENERGY_J ? ue : ue + (double)sh * width(g.dxh[0], i) * width(g.dxh[1], j) * width(g.dxh[2], k);
+  }
+}
+
+// s = E(+1) x H with both width-weighted (fdtd/energy.py:10-87): s_x = ey(i+1) hz - ez(i+1) hy
+// and cyclically, E taken at the next cell along the component's axis (periodic), in
+// double, stored in the field precision.
+template <typename T>
+__global__ __launch_bounds__(kFB) void k_poynting(RGrid g, const T* __restrict__ e, const T* __restrict__ h,
+                                                  T* __restrict__ s) {
+#pragma clang fp contract(off)
+  const int nc = g.nx * g.ny * g.nz;
+  for (int cell = blockIdx.x * kFB + threadIdx.x; cell < nc; cell += gridDim.x * kFB) {
+    const int k = cell % g.nz, t = cell / g.nz, j = t % g.ny, i = t / g.ny;
+    const int sx = (i + 1 == g.nx ? -i : 1) * g.ny * g.nz;
+    const int sy = (j + 1 == g.ny ? -j : 1) * g.nz;
+    const int sz = (k + 1 == g.nz ? -k : 1);
+    const double wex = width(g.dxe[0], i), wey = width(g.dxe[1], j), wez = width(g.dxe[2], k);
+    const double hx = (double)h[cell] * width(g.dxh[0], i);
+    const double hy = (double)h[nc + cell] * width(g.dxh[1], j);
+    const double hz = (double)h[2 * nc + cell] * width(g.dxh[2], k);
+    const double s0 = ((double)e[nc + cell + sx] * wey) * hz - ((double)e[2 * nc + cell + sx] * wez) * hy;
+    const double s1 = ((double)e[2 * nc + cell + sy] * wez) * hx - ((double)e[cell + sy] * wex) * hz;
+    const double s2 = ((double)e[cell + sz] * wex) * hy - ((double)e[nc + cell + sz] * wey) * hx;
+    s[cell] = (T)s0;
+    s[nc + cell] = (T)s1;
+    s[2 * nc + cell] = (T)s2;
+  }
+}
+
+// ds = sum_a (s_a - s_a(previous cell along a)), unit spacing (deriv_back() without
+// widths, fdtd/energy.py:90-125), in the field precision.
+template <typename T>
+__global__ __launch_bounds__(kFB) void k_poynting_div(RGrid g, const T* __restrict__ s, T* __restrict__ ds) {
+#pragma clang fp contract(off)
+  const int nc = g.nx * g.ny * g.nz;
+  for (int cell = blockIdx.x * kFB + threadIdx.x; cell < nc; cell += gridDim.x * kFB) {
+    const int k = cell % g.nz, t = cell / g.nz, j = t % g.ny, i = t / g.ny;
+    const int sx = (i == 0 ? g.nx - 1 : -1) * g.ny * g.nz;   // i-1 (periodic)
+    const int sy = (j == 0 ? g.ny - 1 : -1) * g.nz;
+    const int sz = (k == 0 ? g.nz - 1 : -1);
+    const T dx = s[cell] - s[cell + sx];
+    const T dy = s[nc + cell] - s[nc + cell + sy];
+    const T dz = s[2 * nc + cell] - s[2 * nc + cell + sz];
+    ds[cell] = dx + dy + dz;
+  }
+}
+
 bool rgrid(const mx_fdtd_grid* g, RGrid* v, std::string* why) {
   if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) { *why = "bad grid"; return false; }
   if (3LL * g->nx * g->ny * g->nz >= (1LL << 31)) { *why = "grid too large (3N must be < 2^31)"; return false; }
@@ -484,6 +566,37 @@ hipError_t launch_conducting(const RGrid& v, int prec, int dir, int pol, int is_
   else
     hipLaunchKernelGGL(k_conducting<float>, dim3(grid), dim3(kFB), 0, s, v.nx, v.ny, v.nz, dir, pol, is_h,
                        (float*)f);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const RGrid& v, int prec, int form, double dt, const void* const* f, const void* eps,
+                         double eps_s, const void* mu, double mu_s, double* out, hipStream_t s) {
+  if (prec == MX_REAL64)
+    hipLaunchKernelGGL(k_energy<double>, dim3(blocks(v)), dim3(kFB), 0, s, v, form, dt, (const double*)f[0],
+                       (const double*)f[1], (const double*)f[2], (const double*)f[3], (const double*)eps, eps_s,
+                       (const double*)mu, mu_s, out);
+  else
+    hipLaunchKernelGGL(k_energy<float>, dim3(blocks(v)), dim3(kFB), 0, s, v, form, (float)dt, (const float*)f[0],
+                       (const float*)f[1], (const float*)f[2], (const float*)f[3], (const float*)eps, (float)eps_s,
+                       (const float*)mu, (float)mu_s, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_poynting(const RGrid& v, int prec, const void* e, const void* h, void* out, hipStream_t s) {
+  if (prec == MX_REAL64)
+    hipLaunchKernelGGL(k_poynting<double>, dim3(blocks(v)), dim3(kFB), 0, s, v, (const double*)e, (const double*)h,
+                       (double*)out);
+  else
+    hipLaunchKernelGGL(k_poynting<float>, dim3(blocks(v)), dim3(kFB), 0, s, v, (const float*)e, (const float*)h,
+                       (float*)out);
+  return hipGetLastError();
+}
+
+hipError_t launch_poynting_div(const RGrid& v, int prec, const void* sv, void* ds, hipStream_t s) {
+  if (prec == MX_REAL64)
+    hipLaunchKernelGGL(k_poynting_div<double>, dim3(blocks(v)), dim3(kFB), 0, s, v, (const double*)sv, (double*)ds);
+  else
+    hipLaunchKernelGGL(k_poynting_div<float>, dim3(blocks(v)), dim3(kFB), 0, s, v, (const float*)sv, (float*)ds);
   return hipGetLastError();
 }
 
@@ -650,6 +763,37 @@ int mx_fdtd_conducting_boundary(const mx_fdtd_grid* g, int32_t prec, int32_t dir
   const hipError_t err = mx::launch_conducting(v, prec, direction, polarity, field == MX_FIELD_H, f,
                                                reinterpret_cast<hipStream_t>(stream));
   return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_conducting_boundary: ") + hipGetErrorString(err));
+}
+
+int mx_fdtd_energy(const mx_fdtd_grid* g, int32_t prec, int32_t form, double dt, const void* const* f,
+                   const void* eps, double eps_scalar, const void* mu, double mu_scalar, double* out, void* stream) {
+  static const int kNeeds[6] = {2, 3, 3, 4, 4, 2};   // fields each form reads
+  mx::RGrid v;
+  if (!f) return mx::set_error("mx_fdtd_energy: null field table");
+  if (mx::check_args(g, prec, f[0], out, &v, "mx_fdtd_energy")) return -1;
+  if (form < MX_ENERGY_DXMUL || form > MX_ENERGY_J) return mx::set_error("mx_fdtd_energy: unknown form");
+  for (int q = 0; q < kNeeds[form]; ++q)
+    if (!f[q]) return mx::set_error("mx_fdtd_energy: null field for this form");
+  if ((form == MX_ENERGY_H2E || form == MX_ENERGY_E2H) && !(dt != 0.0)) return mx::set_error("mx_fdtd_energy: dt must be nonzero");
+  const void* fs[4] = {f[0], f[1], kNeeds[form] > 2 ? f[2] : nullptr, kNeeds[form] > 3 ? f[3] : nullptr};
+  const hipError_t err = mx::launch_energy(v, prec, form, dt, fs, eps, eps_scalar, mu, mu_scalar, out,
+                                           reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_energy: ") + hipGetErrorString(err));
+}
+
+int mx_fdtd_poynting(const mx_fdtd_grid* g, int32_t prec, const void* e, const void* h, void* s, void* stream) {
+  mx::RGrid v;
+  if (mx::check_args(g, prec, e, h, &v, "mx_fdtd_poynting")) return -1;
+  if (!s) return mx::set_error("mx_fdtd_poynting: null output");
+  const hipError_t err = mx::launch_poynting(v, prec, e, h, s, reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_poynting: ") + hipGetErrorString(err));
+}
+
+int mx_fdtd_poynting_divergence(const mx_fdtd_grid* g, int32_t prec, const void* s, void* ds, void* stream) {
+  mx::RGrid v;
+  if (mx::check_args(g, prec, s, ds, &v, "mx_fdtd_poynting_divergence")) return -1;
+  const hipError_t err = mx::launch_poynting_div(v, prec, s, ds, reinterpret_cast<hipStream_t>(stream));
+  return err == hipSuccess ? 0 : mx::set_error(std::string("mx_fdtd_poynting_divergence: ") + hipGetErrorString(err));
 }
 
 }  // extern "C"

@@ -20,7 +20,8 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdmath/functional.py             curl_forward / curl_back (callables, dx None = 1)
   meanas/fdtd/base.py:13-125              maxwell_e / maxwell_h
   meanas/fdtd/energy.py:10-295            poynting, poynting_divergence, energy_hstep,
-                                          energy_estep, delta_energy_j, dxmul
+                                          energy_estep, delta_energy_h2e, delta_energy_e2h,
+                                          delta_energy_j, dxmul (the same expression order)
   meanas/fdfd/operators.py:366-450        e_tfsf_source / e_boundary_source
   meanas/fdfd/waveguide_3d.py:113-156,206-252  compute_source / expand_e
   meanas/fdtd/pml.py:23-230               cpml_params / updates_with_cpml
@@ -61,6 +62,9 @@ How each function is pinned (tests/test_oracle.py):
                        restated reference tests of meanas/test/test_fdtd.py (initial
                        fields and energy, energy conservation, Poynting divergence and
                        Poynting planes) over the reference's fixture grid.
+                       delta_energy_h2e / _e2h, which those tests do not call: dt times
+                       each equals the difference of the energies at its two ends
+                       (energy_estep - energy_hstep, and the reverse), by linearity of dxmul.
   sources           -- e_tfsf_source, e_boundary_source, expand_e and compute_source
                        pinned together by the restated one-way PML test
                        (meanas/test/test_fdfd_pml.py:14-28): the source launches a wave
@@ -520,53 +524,70 @@ def maxwell_h(dt: float, dxes=None) -> Callable:
     return mh
 
 
-def _grid_weights(dx):
-    return dx[0][:, None, None] * dx[1][None, :, None] * dx[2][None, None, :]
+def _weigh(u, dx):
+    """u * dx_x * dx_y * dx_z, one axis at a time, as the reference's expressions multiply."""
+    return u * dx[0][:, None, None] * dx[1][None, :, None] * dx[2][None, None, :]
+
+
+def _unit_dxes(dxes):
+    return [[numpy.ones(1)] * 3] * 2 if dxes is None else dxes
 
 
 def fdtd_poynting(e, h, dxes=None):
-    """E x H with E shifted forward, width-weighted (fdtd/energy.py:10-75)."""
-    if dxes is None:
-        dxes = [[numpy.ones(1)] * 3] * 2
+    """E x H with E shifted forward, width-weighted (fdtd/energy.py:10-87); the products
+    are float64 (the widths promote) and the result has e's dtype."""
+    dxes = _unit_dxes(dxes)
     ex, ey, ez = (e[c] * dxes[0][c].reshape([-1 if b == c else 1 for b in range(3)]) for c in range(3))
     hx, hy, hz = (h[c] * dxes[1][c].reshape([-1 if b == c else 1 for b in range(3)]) for c in range(3))
-    return numpy.stack([numpy.roll(ey, -1, 0) * hz - numpy.roll(ez, -1, 0) * hy,
-                        numpy.roll(ez, -1, 1) * hx - numpy.roll(ex, -1, 1) * hz,
-                        numpy.roll(ex, -1, 2) * hy - numpy.roll(ey, -1, 2) * hx])
+    s = numpy.empty_like(e)
+    s[0] = numpy.roll(ey, -1, 0) * hz - numpy.roll(ez, -1, 0) * hy
+    s[1] = numpy.roll(ez, -1, 1) * hx - numpy.roll(ex, -1, 1) * hz
+    s[2] = numpy.roll(ex, -1, 2) * hy - numpy.roll(ey, -1, 2) * hx
+    return s
 
 
 def poynting_divergence(s=None, *, e=None, h=None, dxes=None):
     """Backward-difference divergence of S with unit spacing, as the reference computes it
-    (deriv_back() without widths, fdtd/energy.py:78-110)."""
+    (deriv_back() without widths, fdtd/energy.py:90-125)."""
     if s is None:
         s = fdtd_poynting(e, h, dxes)
-    return sum(s[a] - numpy.roll(s[a], 1, axis=a) for a in range(3))
+    d = [s[a] - numpy.roll(s[a], 1, axis=a) for a in range(3)]
+    return d[0] + d[1] + d[2]
 
 
 def dxmul(ee, hh, epsilon=None, mu=None, dxes=None):
-    """sum_c ee eps dV_e + sum_c hh mu dV_h (fdtd/energy.py:268-295)."""
+    """sum_c ee eps dV_e + sum_c hh mu dV_h (fdtd/energy.py:273-295)."""
     epsilon = 1 if epsilon is None else epsilon
     mu = 1 if mu is None else mu
-    if dxes is None:
-        dxes = [[numpy.ones(1)] * 3] * 2
-    return (ee * epsilon).sum(axis=0) * _grid_weights(dxes[0]) + (hh * mu).sum(axis=0) * _grid_weights(dxes[1])
+    dxes = _unit_dxes(dxes)
+    return _weigh((ee * epsilon).sum(axis=0), dxes[0]) + _weigh((hh * mu).sum(axis=0), dxes[1])
 
 
 def energy_hstep(e0, h1, e2, epsilon=None, mu=None, dxes=None):
-    """Energy at an H step: dxmul(e0 e2, h1 h1) (fdtd/energy.py:113-140)."""
+    """Energy at an H step: dxmul(e0 e2, h1 h1) (fdtd/energy.py:128-153)."""
     return dxmul(e0 * e2, h1 * h1, epsilon, mu, dxes)
 
 
 def energy_estep(h0, e1, h2, epsilon=None, mu=None, dxes=None):
-    """Energy at an E step: dxmul(e1 e1, h0 h2) (fdtd/energy.py:143-170)."""
+    """Energy at an E step: dxmul(e1 e1, h0 h2) (fdtd/energy.py:156-181)."""
     return dxmul(e1 * e1, h0 * h2, epsilon, mu, dxes)
 
 
+def delta_energy_h2e(dt, e0, h1, e2, h3, epsilon=None, mu=None, dxes=None):
+    """Energy change from h1 to e2: dxmul(e2 (e2 - e0) / dt, h1 (h3 - h1) / dt)
+    (fdtd/energy.py:184-214)."""
+    return dxmul(e2 * (e2 - e0) / dt, h1 * (h3 - h1) / dt, epsilon, mu, dxes)
+
+
+def delta_energy_e2h(dt, h0, e1, h2, e3, epsilon=None, mu=None, dxes=None):
+    """Energy change from e1 to h2: dxmul(e1 (e3 - e1) / dt, h2 (h2 - h0) / dt)
+    (fdtd/energy.py:217-247)."""
+    return dxmul(e1 * (e3 - e1) / dt, h2 * (h2 - h0) / dt, epsilon, mu, dxes)
+
+
 def delta_energy_j(j0, e1, dxes=None):
-    """(j0 . e1) dV_e (fdtd/energy.py:239-265)."""
-    if dxes is None:
-        dxes = [[numpy.ones(1)] * 3] * 2
-    return (j0 * e1).sum(axis=0) * _grid_weights(dxes[0])
+    """(j0 . e1) dV_e (fdtd/energy.py:250-270)."""
+    return _weigh((j0 * e1).sum(axis=0), _unit_dxes(dxes)[0])
 
 
 # ---------------------------------------------------------------------------

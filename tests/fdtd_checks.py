@@ -1,7 +1,9 @@
 """The reference's FDTD tests (meanas/test/test_fdtd.py) restated over its fixture grid
 (meanas/test/conftest.py), with the time stepping passed in, so the same checks run on
-the oracle's updates and on the GPU's.  The energy functions are the oracle's."""
+the oracle's updates and on the GPU's.  The energy functions the checks use are the
+oracle's unless the simulation names others (`TDResult.en`)."""
 from dataclasses import dataclass, field
+from types import SimpleNamespace
 
 import numpy
 
@@ -19,6 +21,12 @@ class TDResult:
     es: list = field(default_factory=list)
     hs: list = field(default_factory=list)
     js: list = field(default_factory=list)
+    en: object = None   # the energy functions the checks use (the oracle's unless set)
+
+
+ORACLE_ENERGY = SimpleNamespace(energy_hstep=ref.energy_hstep, energy_estep=ref.energy_estep,
+                                delta_energy_j=ref.delta_energy_j, poynting_divergence=ref.poynting_divergence,
+                                poynting=ref.fdtd_poynting)
 
 
 def cases():
@@ -97,36 +105,40 @@ def check_initial_fields(sim):
     assert not h0.any()
 
 
+def E(sim):
+    return sim.en or ORACLE_ENERGY
+
+
 def check_initial_energy(sim):
     j0, e0, h0, h1 = sim.js[0], sim.es[0], sim.hs[0], sim.hs[1]
     dV = numpy.prod(numpy.meshgrid(*sim.dxes[0], indexing='ij'), axis=0)
     u0 = (j0 * j0.conj() / sim.epsilon * dV).sum(axis=0)
-    close(ref.energy_estep(h0=h0, e1=e0, h2=h1, dxes=sim.dxes, epsilon=sim.epsilon), u0)
-    close(ref.delta_energy_j(j0=j0, e1=e0, dxes=sim.dxes), u0)
+    close(E(sim).energy_estep(h0=h0, e1=e0, h2=h1, dxes=sim.dxes, epsilon=sim.epsilon), u0)
+    close(E(sim).delta_energy_j(j0=j0, e1=e0, dxes=sim.dxes), u0)
 
 
 def check_energy_conservation(sim):
-    u = ref.delta_energy_j(j0=sim.js[0], e1=sim.es[0], dxes=sim.dxes).sum()
+    u = E(sim).delta_energy_j(j0=sim.js[0], e1=sim.es[0], dxes=sim.dxes).sum()
     for ii in range(1, 8):
-        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
-        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
-        u += ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes).sum()
+        u_h = E(sim).energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = E(sim).energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        u += E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes).sum()
         close(u_h.sum(), u)
-        u += ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes).sum()
+        u += E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes).sum()
         close(u_e.sum(), u)
 
 
 def check_poynting_divergence(sim):
     u_prev = None
     for ii in range(1, 8):
-        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
-        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
-        dj_b = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
-        close(u_e - u_h - dj_b, -sim.dt * ref.poynting_divergence(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes))
+        u_h = E(sim).energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = E(sim).energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        dj_b = E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
+        close(u_e - u_h - dj_b, -sim.dt * E(sim).poynting_divergence(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes))
         if u_prev is not None:
-            dj_a = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
+            dj_a = E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
             close(u_h - u_prev - dj_a,
-                  -sim.dt * ref.poynting_divergence(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes))
+                  -sim.dt * E(sim).poynting_divergence(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes))
         u_prev = u_e
 
 
@@ -137,16 +149,16 @@ def check_poynting_planes(sim):
     mx, my, mz = (numpy.roll(mask, -1, axis=a) for a in range(3))
     u_prev = None
     for ii in range(1, 8):
-        u_h = ref.energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
-        u_e = ref.energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
-        dj_b = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
-        s = -ref.fdtd_poynting(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
+        u_h = E(sim).energy_hstep(e0=sim.es[ii - 1], h1=sim.hs[ii], e2=sim.es[ii], dxes=sim.dxes, epsilon=sim.epsilon)
+        u_e = E(sim).energy_estep(h0=sim.hs[ii], e1=sim.es[ii], h2=sim.hs[ii + 1], dxes=sim.dxes, epsilon=sim.epsilon)
+        dj_b = E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii], dxes=sim.dxes)
+        s = -E(sim).poynting(e=sim.es[ii], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
         planes = [s[0, mask].sum(), -s[0, mx].sum(), s[1, mask].sum(), -s[1, my].sum(),
                   s[2, mask].sum(), -s[2, mz].sum()]
         close(sum(planes), (u_e - u_h - dj_b)[mask])
         if u_prev is not None:
-            dj_a = ref.delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
-            s = -ref.fdtd_poynting(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
+            dj_a = E(sim).delta_energy_j(j0=sim.js[ii], e1=sim.es[ii - 1], dxes=sim.dxes)
+            s = -E(sim).poynting(e=sim.es[ii - 1], h=sim.hs[ii], dxes=sim.dxes) * sim.dt
             planes = [s[0, mask].sum(), -s[0, mx].sum(), s[1, mask].sum(), -s[1, my].sum(),
                       s[2, mask].sum(), -s[2, mz].sum()]
             close(sum(planes), (u_h - u_prev - dj_a)[mask])

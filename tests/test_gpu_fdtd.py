@@ -244,3 +244,76 @@ def test_conducting_boundary_numpy_and_errors():
         fdtd.conducting_boundary(3, 1)
     with pytest.raises(Exception, match='Bad polarity'):
         fdtd.conducting_boundary(1, 0)
+
+
+# ---- energy and Poynting functions (meanas/fdtd/energy.py) -------------------------
+def _gpu_energy():
+    from types import SimpleNamespace
+    from meanas_amd import fdtd
+    return SimpleNamespace(energy_hstep=fdtd.energy_hstep, energy_estep=fdtd.energy_estep,
+                           delta_energy_j=fdtd.delta_energy_j, poynting_divergence=fdtd.poynting_divergence,
+                           poynting=fdtd.poynting)
+
+
+@pytest.mark.parametrize('case', fc.cases())
+def test_reference_fdtd_checks_with_gpu_energy(case):
+    """The reference's FDTD checks (initial energy, energy conservation, Poynting
+    divergence and planes) on GPU-stepped fields, with the GPU's energy and Poynting
+    functions as the checker."""
+    sim = fc.make_sim(case, gpu_step)
+    sim.en = _gpu_energy()
+    for check in fc.CHECKS:
+        check(sim)
+
+
+@pytest.mark.parametrize('dtype,tol', [(numpy.float64, 1e-14), (numpy.float32, 1e-6)])
+@pytest.mark.parametrize('shape', [(5, 6, 7), (1, 4, 9), (8, 1, 3)])
+@pytest.mark.parametrize('widths', [True, False])
+@pytest.mark.parametrize('mats', ['field', 'scalar', None])
+def test_energy_functions_match_oracle(dtype, tol, shape, widths, mats):
+    """Every function of energy.py against the oracle's, which follows the reference's
+    expression order (products and component sums in the field precision, widths in
+    float64, one axis at a time)."""
+    from meanas_amd import fdtd
+    rng = numpy.random.default_rng(len(shape) * 7 + shape[0])
+    f = [rng.standard_normal((3, *shape)).astype(dtype) for _ in range(4)]
+    dxes = [[0.5 + rng.random(n) for n in shape] for _ in range(2)] if widths else None
+    eps = {'field': (1 + rng.random((3, *shape))).astype(dtype), 'scalar': 2.5, None: None}[mats]
+    mu = {'field': (1 + rng.random((3, *shape))).astype(dtype), 'scalar': 1.5, None: None}[mats]
+    dt = 0.37
+    kw = dict(epsilon=eps, mu=mu, dxes=dxes)
+    dxes_given = dxes if dxes is not None else [[numpy.ones(n) for n in shape]] * 2
+    pairs = {
+        'hstep': (fdtd.energy_hstep(f[0], f[1], f[2], **kw), ref.energy_hstep(f[0], f[1], f[2], **kw)),
+        'estep': (fdtd.energy_estep(f[0], f[1], f[2], **kw), ref.energy_estep(f[0], f[1], f[2], **kw)),
+        'h2e': (fdtd.delta_energy_h2e(dt, *f, **kw), ref.delta_energy_h2e(dt, *f, **kw)),
+        'e2h': (fdtd.delta_energy_e2h(dt, *f, **kw), ref.delta_energy_e2h(dt, *f, **kw)),
+        'j': (fdtd.delta_energy_j(f[0], f[1], dxes=dxes), ref.delta_energy_j(f[0], f[1], dxes=dxes)),
+        'dxmul': (fdtd.dxmul(f[0], f[1], **kw), ref.dxmul(f[0], f[1], **kw)),
+        'poynting': (fdtd.poynting(f[0], f[1], dxes), ref.fdtd_poynting(f[0], f[1], dxes)),
+        'div_s': (fdtd.poynting_divergence(f[2]), ref.poynting_divergence(f[2])),
+        'div_eh': (fdtd.poynting_divergence(e=f[0], h=f[1], dxes=dxes_given),
+                   ref.poynting_divergence(e=f[0], h=f[1], dxes=dxes_given)),
+    }
+    for name, (got, want) in pairs.items():
+        assert isinstance(got, numpy.ndarray) and got.dtype == want.dtype and got.shape == want.shape, name
+        scale = numpy.abs(want).max()
+        numpy.testing.assert_allclose(got, want, rtol=tol, atol=tol * scale, err_msg=name)
+
+
+def test_energy_functions_on_device_tensors():
+    """CUDA tensors in, CUDA tensors out, equal to the numpy path; mixed dtypes refused."""
+    from meanas_amd import fdtd
+    rng = numpy.random.default_rng(3)
+    e, h, e2 = (rng.standard_normal((3, 6, 5, 4)) for _ in range(3))
+    dev = [torch.from_numpy(a).cuda() for a in (e, h, e2)]
+    u = fdtd.energy_hstep(*dev, epsilon=2.0)
+    assert isinstance(u, torch.Tensor) and u.is_cuda and u.dtype == torch.float64
+    numpy.testing.assert_array_equal(u.cpu().numpy(), fdtd.energy_hstep(e, h, e2, epsilon=2.0))
+    s = fdtd.poynting(dev[0], dev[1])
+    assert isinstance(s, torch.Tensor) and s.shape == (3, 6, 5, 4)
+    numpy.testing.assert_array_equal(s.cpu().numpy(), fdtd.poynting(e, h))
+    with pytest.raises(ValueError):
+        fdtd.energy_estep(e, h.astype(numpy.float32), e2)
+    with pytest.raises(AssertionError):     # as the reference: e, h and dxes, or s
+        fdtd.poynting_divergence(e=e, h=h)

@@ -74,6 +74,32 @@ def test_conducting_boundary_checks_arguments_on_the_host():
         assert msg in L.mx_last_error(), (args, L.mx_last_error())
 
 
+def test_energy_entry_points_check_arguments_on_the_host():
+    """mx_fdtd_energy and the Poynting entry points refuse bad arguments before any
+    device call (each call below fails a check; no buffer reaches a kernel)."""
+    from meanas_amd import _lib
+    L = _lib.lib()
+    buf = numpy.zeros(3 * 4 * 4 * 2)
+    out = numpy.zeros(4 * 4 * 2)
+    grid = _lib.MxFdtdGrid(4, 4, 2, 0, (ctypes.c_void_p * 3)(), (ctypes.c_void_p * 3)())
+    p = buf.ctypes.data
+    table = (ctypes.c_void_p * 4)(p, p, None, None)
+
+    def energy(form, tab=table, dt=0.5, o=out.ctypes.data):
+        return L.mx_fdtd_energy(ctypes.byref(grid), _lib.MX_REAL64, form, dt, tab, None, 1.0, None, 1.0, o, None)
+    for rc_args, msg in [((9,), b'unknown form'), ((_lib.MX_ENERGY_HSTEP,), b'null field for this form'),
+                         ((_lib.MX_ENERGY_DXMUL, None), b'null field table'),
+                         ((_lib.MX_ENERGY_DXMUL, table, 0.5, None), b'null field')]:
+        assert energy(*rc_args) != 0
+        assert msg in L.mx_last_error(), (rc_args, L.mx_last_error())
+    full = (ctypes.c_void_p * 4)(p, p, p, p)
+    assert energy(_lib.MX_ENERGY_H2E, full, 0.0) != 0 and b'dt' in L.mx_last_error()
+    assert L.mx_fdtd_poynting(ctypes.byref(grid), _lib.MX_REAL64, p, p, None, None) != 0
+    assert b'null output' in L.mx_last_error()
+    assert L.mx_fdtd_poynting_divergence(ctypes.byref(grid), 7, p, p, None) != 0
+    assert b'prec' in L.mx_last_error()
+
+
 def test_product_scpml_matches_oracle():
     from meanas_amd.fdfd import scpml
     shape = (30, 1, 41)

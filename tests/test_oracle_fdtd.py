@@ -174,3 +174,23 @@ def test_reference_indexing_needs_tuples():
         ref.conducting_boundary(3, 1)
     with pytest.raises(Exception, match='Bad polarity'):
         ref.conducting_boundary(0, 0)
+
+
+def test_energy_deltas_are_differences_of_step_energies():
+    """dt delta_energy_h2e(e0, h1, e2, h3) = energy_estep(h1, e2, h3) - energy_hstep(e0, h1, e2)
+    and dt delta_energy_e2h(h0, e1, h2, e3) = energy_hstep(e1, h2, e3) - energy_estep(h0, e1, h2):
+    dxmul is linear in each argument.  A swapped field or sign in either delta fails."""
+    rng = numpy.random.default_rng(8)
+    shape = (4, 5, 3)
+    f = [rng.standard_normal((3, *shape)) for _ in range(4)]
+    eps, mu = 1 + rng.random((3, *shape)), 1 + rng.random((3, *shape))
+    dxes = [[0.5 + rng.random(n) for n in shape] for _ in range(2)]
+    dt = 0.37
+    kw = dict(epsilon=eps, mu=mu, dxes=dxes)
+    h2e = ref.delta_energy_h2e(dt, *f, **kw) * dt
+    numpy.testing.assert_allclose(h2e, ref.energy_estep(f[1], f[2], f[3], **kw) - ref.energy_hstep(f[0], f[1], f[2], **kw),
+                                  rtol=1e-12, atol=1e-12)
+    e2h = ref.delta_energy_e2h(dt, *f, **kw) * dt
+    numpy.testing.assert_allclose(e2h, ref.energy_hstep(f[1], f[2], f[3], **kw) - ref.energy_estep(f[0], f[1], f[2], **kw),
+                                  rtol=1e-12, atol=1e-12)
+    assert ref.fdtd_poynting(f[0].astype(numpy.float32), f[1].astype(numpy.float32), dxes).dtype == numpy.float32
