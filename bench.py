@@ -197,11 +197,41 @@ def fdtd_workload(args):
         cev[1].record(stream)
         torch.cuda.synchronize()
         us_cpml = cev[0].elapsed_time(cev[1]) * 1e3 / csteps
+        # analysis calls on the same fields: the energy at an H step (e0, h1, e2 and eps
+        # read, N float64 written), the Poynting vector (e, h read, s written) and one
+        # conducting face (E and H); per call, including the upload of the widths
+        e2 = e.clone()
+        en_, hn_ = fdtd.conducting_boundary(0, -1)
+        fdtd.energy_hstep(e, h, e2, epsilon=eps, dxes=dxes)
+        fdtd.poynting(e, h, dxes)
+        aev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        areps = 10
+        aev[0].record(stream)
+        for _ in range(areps):
+            fdtd.energy_hstep(e, h, e2, epsilon=eps, dxes=dxes)
+        aev[1].record(stream)
+        for _ in range(areps):
+            fdtd.poynting(e, h, dxes)
+        aev[2].record(stream)
+        for _ in range(areps):
+            en_(e)
+            hn_(h)
+        aev[3].record(stream)
+        torch.cuda.synchronize()
+        us_en = aev[0].elapsed_time(aev[1]) * 1e3 / areps
+        us_s = aev[1].elapsed_time(aev[2]) * 1e3 / areps
+        us_cb = aev[2].elapsed_time(aev[3]) * 1e3 / areps
+        b_en = 12 * elem * cells + 8 * cells
+        b_s = 9 * elem * cells
+        del e2
         out[name] = dict(step_us=round(us_step, 2), mcell_updates_per_s=round(cells / us_step, 1),
                          h_update=dict(us=round(us_h, 2), bytes=b_h, gbs=round(b_h / us_h / 1e3, 1)),
                          e_update=dict(us=round(us_e, 2), bytes=b_e, gbs=round(b_e / us_e / 1e3, 1)),
                          cpml_step=dict(us=round(us_cpml, 2), mcell_updates_per_s=round(cells / us_cpml, 1),
-                                        layers='10 cells on all six faces, psi float32'))
+                                        layers='10 cells on all six faces, psi float32'),
+                         energy_hstep=dict(us=round(us_en, 2), bytes=b_en, gbs=round(b_en / us_en / 1e3, 1)),
+                         poynting=dict(us=round(us_s, 2), bytes=b_s, gbs=round(b_s / us_s / 1e3, 1)),
+                         conducting_face_e_and_h=dict(us=round(us_cb, 2)))
         log(f'fdtd {name}: {out[name]}')
         del uE, uH
         del e, h, eps

@@ -18,11 +18,29 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy
 import torch
 
 from .. import _lib
 from ..fdfd.device import torch_stream_handle
 from .base import _Field, _Grid, _material, _prec
+
+
+_GRIDS: dict = {}
+
+
+def _grid(dxes, shape, device) -> _Grid:
+    """Device widths for `dxes`, reused across calls with the same widths.  The key is
+    their content, since callers may change dxes in place between calls."""
+    key = (tuple(shape), str(device))
+    if dxes is not None:
+        key += tuple(numpy.ascontiguousarray(numpy.real(d), dtype=numpy.float64).tobytes() for side in dxes for d in side)
+    g = _GRIDS.get(key)
+    if g is None:
+        if len(_GRIDS) >= 16:
+            _GRIDS.clear()
+        g = _GRIDS[key] = _Grid(dxes, shape, device)
+    return g
 
 
 def _device_fields(fields, device):
@@ -46,7 +64,7 @@ def _energy(form: int, dt: float, fields, epsilon, mu, dxes, device):
     ts, host = _device_fields(fields, device)
     like = ts[0]
     shape = tuple(like.shape[1:])
-    grid = _Grid(dxes, shape, like.device)
+    grid = _grid(dxes, shape, like.device)
     me, es = _material(epsilon, like)
     mm, ms = _material(mu, like)
     out = torch.empty(shape, dtype=torch.float64, device=like.device)
@@ -60,7 +78,7 @@ def poynting(e, h, dxes=None, *, device='cuda'):
     """s_x = ey(i+1) hz - ez(i+1) hy and cyclically, fields times their widths
     (energy.py:10-87)."""
     (et, ht), host = _device_fields((e, h), device)
-    grid = _Grid(dxes, tuple(et.shape[1:]), et.device)
+    grid = _grid(dxes, tuple(et.shape[1:]), et.device)
     s = torch.empty_like(et)
     _lib.check(_lib.lib().mx_fdtd_poynting(ctypes.byref(grid.desc), _prec(et.dtype), et.data_ptr(), ht.data_ptr(),
                                            s.data_ptr(), torch_stream_handle()), 'mx_fdtd_poynting')
@@ -77,7 +95,7 @@ def poynting_divergence(s=None, *, e=None, h=None, dxes=None, device='cuda'):
         s = poynting(e, h, dxes=dxes, device=device)
     (st,), host = _device_fields((s,), device)
     shape = tuple(st.shape[1:])
-    grid = _Grid(None, shape, st.device)
+    grid = _grid(None, shape, st.device)
     ds = torch.empty(shape, dtype=st.dtype, device=st.device)
     _lib.check(_lib.lib().mx_fdtd_poynting_divergence(ctypes.byref(grid.desc), _prec(st.dtype), st.data_ptr(),
                                                       ds.data_ptr(), torch_stream_handle()),

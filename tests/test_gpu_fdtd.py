@@ -313,6 +313,12 @@ def test_energy_functions_on_device_tensors():
     s = fdtd.poynting(dev[0], dev[1])
     assert isinstance(s, torch.Tensor) and s.shape == (3, 6, 5, 4)
     numpy.testing.assert_array_equal(s.cpu().numpy(), fdtd.poynting(e, h))
+    dxes = [[0.5 + rng.random(n) for n in (6, 5, 4)] for _ in range(2)]
+    first = fdtd.energy_estep(e, h, e2, dxes=dxes)
+    numpy.testing.assert_allclose(first, ref.energy_estep(e, h, e2, dxes=dxes), rtol=1e-14)
+    dxes[0][1] *= 1.5                        # changed in place: the device widths follow
+    numpy.testing.assert_allclose(fdtd.energy_estep(e, h, e2, dxes=dxes), ref.energy_estep(e, h, e2, dxes=dxes),
+                                  rtol=1e-14)
     with pytest.raises(ValueError):
         fdtd.energy_estep(e, h.astype(numpy.float32), e2)
     with pytest.raises(AssertionError):     # as the reference: e, h and dxes, or s
