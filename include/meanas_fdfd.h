@@ -182,6 +182,13 @@ typedef struct mx_grid {
 int mx_e2h(const mx_grid* g, double omega_re, double omega_im, const void* inv_mu, const uint8_t* pmc,
            const void* e, void* h, void* stream);
 
+/* J from a magnetic current M: J = curl_back(M / mu) / (-i omega), backward differences on
+ * dxes[1].  Replaces fdfd/functional.py:120-151 (m2j).  The sparse form
+ * (fdfd/operators.py:285-307) divides by +i omega instead, so it equals -J.  mu: 3N
+ * complex128 or NULL (mu = 1); m, j: 3N complex128 in vec() order. */
+int mx_m2j(const mx_grid* g, double omega_re, double omega_im, const void* mu, const void* m, void* j,
+           void* stream);
+
 /* The Poynting cross product with the E field shifted one cell forward, both fields
  * weighted by their cell widths: s = E x H as fdfd/functional.py:187-221 computes it
  * (poynting_e_cross_h), equal to poynting_e_cross(e, dxes) @ h of fdfd/operators.py:310-336.

@@ -123,6 +123,8 @@ PROTOTYPES = {
                               c_void_p, c_void_p, c_void_p]),
     'mx_solve': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.c_int32, ctypes.POINTER(MxSystem),
                                 ctypes.c_int32, ctypes.POINTER(MxOptions), ctypes.POINTER(MxStats), c_void_p]),
+    'mx_m2j': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
     'mx_e2h': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p]),
     'mx_poynting_e_cross_h': (ctypes.c_int, [ctypes.POINTER(MxGrid), c_void_p, c_void_p, ctypes.c_int32, c_void_p,

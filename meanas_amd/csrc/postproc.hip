@@ -70,6 +70,32 @@ __global__ __launch_bounds__(kPB) void k_e2h(GridView g, cplx<double> c, const c
   }
 }
 
+// J = curl_back(M / mu) / (-i omega) (fdfd/functional.py:120-151, m2j): backward
+// differences on the H-grid widths, periodic; M / mu is formed at the cell and at the
+// three backward neighbours the curl reads.
+__global__ __launch_bounds__(kPB) void k_m2j(GridView g, cplx<double> c, const cplx<double>* __restrict__ mu,
+                                             const cplx<double>* __restrict__ m, cplx<double>* __restrict__ jout) {
+  const int nc = g.nx * g.ny * g.nz;
+  auto v = [&](int q) { return mu ? cdiv(m[q], mu[q]) : m[q]; };
+  for (int cell = blockIdx.x * kPB + threadIdx.x; cell < nc; cell += gridDim.x * kPB) {
+    int i, j, k;
+    cell_coords(cell, g.ny, g.nz, &i, &j, &k);
+    const int sx = (i == 0 ? g.nx - 1 : -1) * g.ny * g.nz;   // offset of the i-1 neighbour (periodic)
+    const int sy = (j == 0 ? g.ny - 1 : -1) * g.nz;
+    const int sz = (k == 0 ? g.nz - 1 : -1);
+    const cplx<double> vx = v(cell), vy = v(nc + cell), vz = v(2 * nc + cell);
+    const cplx<double> rx = cdiv(cplx<double>{1.0, 0.0}, g.dxh[0][i]);
+    const cplx<double> ry = cdiv(cplx<double>{1.0, 0.0}, g.dxh[1][j]);
+    const cplx<double> rz = cdiv(cplx<double>{1.0, 0.0}, g.dxh[2][k]);
+    const cplx<double> dy_vz = ry * (vz - v(2 * nc + cell + sy)), dz_vy = rz * (vy - v(nc + cell + sz));
+    const cplx<double> dz_vx = rz * (vx - v(cell + sz)), dx_vz = rx * (vz - v(2 * nc + cell + sx));
+    const cplx<double> dx_vy = rx * (vy - v(nc + cell + sx)), dy_vx = ry * (vx - v(cell + sy));
+    jout[cell] = c * (dy_vz - dz_vy);
+    jout[nc + cell] = c * (dz_vx - dx_vz);
+    jout[2 * nc + cell] = c * (dx_vy - dy_vx);
+  }
+}
+
 __global__ __launch_bounds__(kPB) void k_exh(GridView g, int conj_h, const cplx<double>* __restrict__ e,
                                              const cplx<double>* __restrict__ hin, cplx<double>* __restrict__ s) {
   const int nc = g.nx * g.ny * g.nz;
@@ -131,6 +157,23 @@ int mx_e2h(const mx_grid* g, double omega_re, double omega_im, const void* inv_m
                      reinterpret_cast<cplx<double>*>(h));
   const hipError_t err = hipGetLastError();
   return err == hipSuccess ? 0 : set_error(std::string("mx_e2h: ") + hipGetErrorString(err));
+}
+
+int mx_m2j(const mx_grid* g, double omega_re, double omega_im, const void* mu, const void* m, void* j,
+           void* stream) {
+  using namespace mx;
+  GridView v;
+  std::string why;
+  if (!grid_view(g, &v, &why)) return set_error("mx_m2j: " + why);
+  if (!m || !j) return set_error("mx_m2j: null field");
+  const cplx<double> w{omega_re, omega_im};
+  if (w.re == 0.0 && w.im == 0.0) return set_error("mx_m2j: omega = 0");
+  const cplx<double> c = cdiv(cplx<double>{1.0, 0.0}, cplx<double>{w.im, -w.re});   // 1 / (-i omega)
+  hipLaunchKernelGGL(k_m2j, dim3(grid_blocks(v)), dim3(kPB), 0, reinterpret_cast<hipStream_t>(stream), v, c,
+                     reinterpret_cast<const cplx<double>*>(mu), reinterpret_cast<const cplx<double>*>(m),
+                     reinterpret_cast<cplx<double>*>(j));
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? 0 : set_error(std::string("mx_m2j: ") + hipGetErrorString(err));
 }
 
 int mx_poynting_e_cross_h(const mx_grid* g, const void* e, const void* h, int32_t conj_h, void* s, void* stream) {

@@ -6,6 +6,10 @@ Mirrors the matrix-free forms of the reference:
   `H = curl_forward(E) / (-i omega) / mu`, as `meanas/fdfd/functional.py:89-117`
   (`e2h`); the optional `pmc` mask zeroes H components as the sparse form does
   (`meanas/fdfd/operators.py:253-282`).
+- `m2j(omega, dxes, mu=None)` returns `f(M) -> J = curl_back(M / mu) / (-i omega)`, as
+  `meanas/fdfd/functional.py:120-151`.  The reference's sparse form
+  (`meanas/fdfd/operators.py:285-307`) divides by +i omega instead, so its product with
+  M is -J.
 - `poynting_e_cross_h(dxes)` returns `f(E, H) -> E x H` with E shifted one cell forward
   and both fields weighted by their cell widths, as `meanas/fdfd/functional.py:187-221`.
   The time-averaged Poynting vector is `Re(f(E, H.conj())) / 2`; `poynting(E, H, dxes)`
@@ -30,7 +34,7 @@ It also gives the wave operators as callables:
 
 Fields may be given as `(3, nx, ny, nz)` arrays (the functional layout) or as `vec()`
 vectors of length 3N.  numpy inputs give numpy outputs of the same shape; torch tensors
-stay on the device.  The kernels run through the C ABI (`mx_e2h`,
+stay on the device.  The kernels run through the C ABI (`mx_e2h`, `mx_m2j`,
 `mx_poynting_e_cross_h`, `mx_map`, `mx_apply`).
 """
 from __future__ import annotations
@@ -106,6 +110,27 @@ def e2h(omega: complex, dxes, mu=None, pmc=None, device: str | torch.device = 'c
                                      mask.data_ptr() if mask is not None else None,
                                      et.data_ptr(), h.data_ptr(), torch_stream_handle()), 'mx_e2h')
         return _from_device(h, like)
+    return f
+
+
+def m2j(omega: complex, dxes, mu=None, device: str | torch.device = 'cuda') -> Callable:
+    """`f(M) -> J = curl_back(M / mu) / (-i omega)` (fdfd/functional.py:120-151)."""
+    grid = _Grid(dxes, device)
+    mu_t = None
+    if mu is not None:
+        m = numpy.asarray(mu, dtype=numpy.complex128).reshape(-1)
+        if m.size != grid.n3:
+            raise ValueError('mu must have 3N entries')
+        mu_t = torch.from_numpy(m.copy()).to(grid.device)
+    w = complex(omega)
+
+    def f(m):
+        mt, like = _to_device(m, grid, 'M')
+        j = torch.empty_like(mt)
+        _lib.check(_lib.lib().mx_m2j(ctypes.byref(grid.desc), w.real, w.imag,
+                                     mu_t.data_ptr() if mu_t is not None else None, mt.data_ptr(), j.data_ptr(),
+                                     torch_stream_handle()), 'mx_m2j')
+        return _from_device(j, like)
     return f
 
 

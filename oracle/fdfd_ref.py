@@ -18,6 +18,7 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdfd/operators.py:126-250        h_full / eh_full
   meanas/fdmath/operators.py:55-99        shift_with_mirror
   meanas/fdmath/functional.py             curl_forward / curl_back (callables, dx None = 1)
+  meanas/fdfd/functional.py:120-151       m2j (and the sparse form, fdfd/operators.py:285-307)
   meanas/fdtd/base.py:13-125              maxwell_e / maxwell_h
   meanas/fdtd/energy.py:10-295            poynting, poynting_divergence, energy_hstep,
                                           energy_estep, delta_energy_h2e, delta_energy_e2h,
@@ -57,6 +58,9 @@ How each function is pinned (tests/test_oracle.py):
   eh_full           -- with H = e2h(E) (no M, no masks) its second block row vanishes
                        and -i omega times the first equals e_full @ E.
   shift_with_mirror -- explicit index table on a small grid (mirror at both edges).
+  m2j               -- plane-wave closed form: with uniform widths, M = a e^{ik.r} gives
+                       J = (kb x a / mu) e^{ik.r} / (-i omega), kb_c = (1 - e^{-i k_c d_c}) / d_c;
+                       the sparse form equals minus the functional one.
   fdtd              -- the functional curls equal the sparse curl_forward/curl_back; the
                        updates and the energy functions are pinned together by the
                        restated reference tests of meanas/test/test_fdtd.py (initial
@@ -502,6 +506,23 @@ def f_curl_back(dx_h=None) -> Callable:
     def curl(h):
         return numpy.stack([d(h[2], 1) - d(h[1], 2), d(h[0], 2) - d(h[2], 0), d(h[1], 0) - d(h[0], 1)])
     return curl
+
+
+def m2j_functional(omega, dxes, mu=None) -> Callable:
+    """J = curl_back(M / mu) / (-i omega) on (3, nx, ny, nz) arrays (fdfd/functional.py:120-151)."""
+    ch = f_curl_back(dxes[1])
+    if mu is None:
+        return lambda m: ch(m) / (-1j * omega)
+    return lambda m: ch(m / mu) / (-1j * omega)
+
+
+def m2j(omega, dxes, mu=None):
+    """Sparse form (fdfd/operators.py:285-307): curl_back / (i omega), times diag(1/mu)
+    on the right.  Its sign is opposite to the functional form's."""
+    op = curl_back(dxes[1]) / (1j * omega)
+    if mu is not None:
+        op = op @ sparse.diags_array(1 / mu)
+    return op
 
 
 def maxwell_e(dt: float, dxes=None) -> Callable:

@@ -477,3 +477,27 @@ def test_eh_full_reduces_to_e_full():
     out = ref.eh_full(omega, dxes, eps, mu) @ numpy.concatenate([e, h])
     numpy.testing.assert_allclose(out[n3:], 0, atol=1e-12)
     numpy.testing.assert_allclose(-1j * omega * out[:n3], ref.e_full(omega, dxes, eps, mu) @ e, rtol=1e-12)
+
+
+def test_m2j_plane_wave_closed_form():
+    """On uniform widths d, M = a e^{i k.r} gives J = (kb x a / mu) e^{i k.r} / (-i omega)
+    with the backward-difference factor kb_c = (1 - e^{-i k_c d_c}) / d_c; the sparse
+    form (fdfd/operators.py:285-307) is minus the functional one."""
+    shape = (6, 5, 4)
+    d = (0.7, 1.3, 0.9)
+    n = (1, 2, 3)                                            # wave numbers in cells
+    k = [2 * numpy.pi * n[a] / (shape[a] * d[a]) for a in range(3)]
+    dxes = [[numpy.full(shape[a], d[a]) for a in range(3)] for _ in range(2)]
+    grids = numpy.meshgrid(*[numpy.arange(shape[a]) * d[a] for a in range(3)], indexing='ij')
+    phase = numpy.exp(1j * sum(k[a] * grids[a] for a in range(3)))
+    amp = numpy.array([0.3 - 0.2j, 1.1, -0.4 + 0.9j])
+    mu0 = 1.7 - 0.1j
+    omega = 0.83 + 0.02j
+    m = amp[:, None, None, None] * phase
+    kb = numpy.array([(1 - numpy.exp(-1j * k[a] * d[a])) / d[a] for a in range(3)])
+    want = numpy.cross(kb, amp / mu0)[:, None, None, None] * phase / (-1j * omega)
+    got = ref.m2j_functional(omega, dxes, numpy.full((3, *shape), mu0))(m)
+    numpy.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+    sp = ref.m2j(omega, dxes, numpy.full(3 * numpy.prod(shape), mu0)) @ ref.vec(m)
+    numpy.testing.assert_allclose(sp, -ref.vec(got), rtol=1e-12, atol=1e-12)
+    numpy.testing.assert_allclose(ref.m2j_functional(omega, dxes)(m), got * mu0, rtol=1e-12, atol=1e-12)
