@@ -196,6 +196,11 @@ int mx_m2j(const mx_grid* g, double omega_re, double omega_im, const void* mu, c
  * that conjugation in the kernel.  e, h, s: 3N complex128. */
 int mx_poynting_e_cross_h(const mx_grid* g, const void* e, const void* h, int32_t conj_h, void* s, void* stream);
 
+/* poynting_h_cross(h, dxes) @ e (fdfd/operators.py:339-363, which builds the sparse
+ * matrix): s = -(E(+1) x H) with the same width weights as mx_poynting_e_cross_h, e
+ * conjugated first when conj_e is nonzero.  h, e, s: 3N complex128 in vec() order. */
+int mx_poynting_h_cross_e(const mx_grid* g, const void* h, const void* e, int32_t conj_e, void* s, void* stream);
+
 /* ---- time domain (FDTD) ------------------------------------------------------------
  * Real fields (3, nx, ny, nz) in vec() order, float32 (MX_REAL32) or float64 (MX_REAL64).
  * Grid: real cell widths dxes[0] (E grid) and dxes[1] (H grid), float64 device arrays of

@@ -127,6 +127,8 @@ PROTOTYPES = {
                               c_void_p]),
     'mx_e2h': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p]),
+    'mx_poynting_h_cross_e': (ctypes.c_int, [ctypes.POINTER(MxGrid), c_void_p, c_void_p, ctypes.c_int32, c_void_p,
+                                             c_void_p]),
     'mx_poynting_e_cross_h': (ctypes.c_int, [ctypes.POINTER(MxGrid), c_void_p, c_void_p, ctypes.c_int32, c_void_p,
                                              c_void_p]),
     'mx_fdtd_h': (ctypes.c_int, [ctypes.POINTER(MxFdtdGrid), ctypes.c_int32, ctypes.c_double, c_void_p, c_void_p,

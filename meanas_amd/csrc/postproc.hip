@@ -119,6 +119,30 @@ __global__ __launch_bounds__(kPB) void k_exh(GridView g, int conj_h, const cplx<
   }
 }
 
+// s = poynting_h_cross(h, dxes) @ e (fdfd/operators.py:339-363): the width-weighted H at
+// the cell against the width-weighted E at the forward neighbour, s = -(E(+1) x H); e is
+// conjugated first when conj_e is set.
+__global__ __launch_bounds__(kPB) void k_hxe(GridView g, int conj_e, const cplx<double>* __restrict__ hin,
+                                             const cplx<double>* __restrict__ e, cplx<double>* __restrict__ s) {
+  const int nc = g.nx * g.ny * g.nz;
+  auto ev = [&](int q) { return conj_e ? conj(e[q]) : e[q]; };
+  for (int cell = blockIdx.x * kPB + threadIdx.x; cell < nc; cell += gridDim.x * kPB) {
+    int i, j, k;
+    cell_coords(cell, g.ny, g.nz, &i, &j, &k);
+    const int sx = (i + 1 == g.nx ? -i : 1) * g.ny * g.nz;
+    const int sy = (j + 1 == g.ny ? -j : 1) * g.nz;
+    const int sz = (k + 1 == g.nz ? -k : 1);
+    const cplx<double> hx = hin[cell] * g.dxh[0][i], hy = hin[nc + cell] * g.dxh[1][j];
+    const cplx<double> hz = hin[2 * nc + cell] * g.dxh[2][k];
+    const cplx<double> ey_x = ev(nc + cell + sx) * g.dxe[1][j], ez_x = ev(2 * nc + cell + sx) * g.dxe[2][k];
+    const cplx<double> ez_y = ev(2 * nc + cell + sy) * g.dxe[2][k], ex_y = ev(cell + sy) * g.dxe[0][i];
+    const cplx<double> ex_z = ev(cell + sz) * g.dxe[0][i], ey_z = ev(nc + cell + sz) * g.dxe[1][j];
+    s[cell] = hy * ez_x - hz * ey_x;
+    s[nc + cell] = hz * ex_y - hx * ez_y;
+    s[2 * nc + cell] = hx * ey_z - hy * ex_z;
+  }
+}
+
 bool grid_view(const mx_grid* g, GridView* v, std::string* why) {
   if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) { *why = "bad grid"; return false; }
   if (3LL * g->nx * g->ny * g->nz * 16 >= (1LL << 31)) { *why = "grid too large (3N complex128 must be < 2 GiB)"; return false; }
@@ -187,6 +211,19 @@ int mx_poynting_e_cross_h(const mx_grid* g, const void* e, const void* h, int32_
                      reinterpret_cast<cplx<double>*>(s));
   const hipError_t err = hipGetLastError();
   return err == hipSuccess ? 0 : set_error(std::string("mx_poynting_e_cross_h: ") + hipGetErrorString(err));
+}
+
+int mx_poynting_h_cross_e(const mx_grid* g, const void* h, const void* e, int32_t conj_e, void* s, void* stream) {
+  using namespace mx;
+  GridView v;
+  std::string why;
+  if (!grid_view(g, &v, &why)) return set_error("mx_poynting_h_cross_e: " + why);
+  if (!e || !h || !s) return set_error("mx_poynting_h_cross_e: null field");
+  hipLaunchKernelGGL(k_hxe, dim3(grid_blocks(v)), dim3(kPB), 0, reinterpret_cast<hipStream_t>(stream), v, (int)conj_e,
+                     reinterpret_cast<const cplx<double>*>(h), reinterpret_cast<const cplx<double>*>(e),
+                     reinterpret_cast<cplx<double>*>(s));
+  const hipError_t err = hipGetLastError();
+  return err == hipSuccess ? 0 : set_error(std::string("mx_poynting_h_cross_e: ") + hipGetErrorString(err));
 }
 
 }  // extern "C"

@@ -17,6 +17,11 @@ Mirrors the matrix-free forms of the reference:
   (`meanas/fdfd/operators.py:310-336`), the form the reference's `test_poynting_planes`
   uses (`meanas/test/test_fdfd.py:21-55`).
 
+- `poynting_h_cross(h, dxes)` stands in for the sparse matrix of
+  `meanas/fdfd/operators.py:339-363`: `poynting_h_cross(h, dxes) @ e` gives
+  `-(E x H)` with the same shift and weights (`.apply(e, conj_e=True)` conjugates E on
+  the device).
+
 It also gives the wave operators as callables:
 
 - `curl_forward(dx_e)` / `curl_back(dx_h)` (`meanas/fdmath/functional.py`): the forward
@@ -35,7 +40,7 @@ It also gives the wave operators as callables:
 Fields may be given as `(3, nx, ny, nz)` arrays (the functional layout) or as `vec()`
 vectors of length 3N.  numpy inputs give numpy outputs of the same shape; torch tensors
 stay on the device.  The kernels run through the C ABI (`mx_e2h`, `mx_m2j`,
-`mx_poynting_e_cross_h`, `mx_map`, `mx_apply`).
+`mx_poynting_e_cross_h`, `mx_poynting_h_cross_e`, `mx_map`, `mx_apply`).
 """
 from __future__ import annotations
 
@@ -147,6 +152,31 @@ def poynting_e_cross_h(dxes, device: str | torch.device = 'cuda') -> Callable:
                    'mx_poynting_e_cross_h')
         return _from_device(s, like)
     return f
+
+
+class _HCross:
+    """`poynting_h_cross(h, dxes) @ e` on the device; H is uploaded once."""
+
+    def __init__(self, h, dxes, device):
+        self.grid = _Grid(dxes, device)
+        self.h, _ = _to_device(h, self.grid, 'H')
+
+    def apply(self, e, conj_e: bool = False):
+        et, like = _to_device(e, self.grid, 'E')
+        s = torch.empty_like(et)
+        _lib.check(_lib.lib().mx_poynting_h_cross_e(ctypes.byref(self.grid.desc), self.h.data_ptr(), et.data_ptr(),
+                                                    1 if conj_e else 0, s.data_ptr(), torch_stream_handle()),
+                   'mx_poynting_h_cross_e')
+        return _from_device(s, like)
+
+    def __matmul__(self, e):
+        return self.apply(e)
+
+
+def poynting_h_cross(h, dxes, device: str | torch.device = 'cuda') -> _HCross:
+    """Stand-in for the sparse `poynting_h_cross(h, dxes)` (fdfd/operators.py:339-363):
+    `P @ e = -(E x H)`, E shifted one cell forward, both fields width-weighted."""
+    return _HCross(h, dxes, device)
 
 
 def poynting(e, h, dxes, device: str | torch.device = 'cuda'):

@@ -13,6 +13,7 @@ Restated from the reference's text (paths relative to the reference root):
   meanas/fdfd/operators.py:99-123         e_full_preconditioners
   meanas/fdfd/operators.py:253-282        e2h
   meanas/fdfd/operators.py:310-336        poynting_e_cross
+  meanas/fdfd/operators.py:339-363        poynting_h_cross
   meanas/fdfd/scpml.py:18-38,98-167       prepare_s_function / stretch_with_scpml
   meanas/fdfd/solvers.py:21-141           _scipy_qmr / generic
   meanas/fdfd/operators.py:126-250        h_full / eh_full
@@ -57,6 +58,8 @@ How each function is pinned (tests/test_oracle.py):
                        ones); a swapped role of eps/mu or pec/pmc fails it.
   eh_full           -- with H = e2h(E) (no M, no masks) its second block row vanishes
                        and -i omega times the first equals e_full @ E.
+  poynting_h_cross  -- P_h(h) @ e = -(P_e(e) @ h) against the pinned poynting_e_cross, on
+                       random complex widths and fields.
   shift_with_mirror -- explicit index table on a small grid (mirror at both edges).
   m2j               -- plane-wave closed form: with uniform widths, M = a e^{ik.r} gives
                        J = (kb x a / mu) e^{ik.r} / (-i omega), kb_c = (1 - e^{-i k_c d_c}) / d_c;
@@ -275,6 +278,22 @@ def poynting_e_cross(e, dxes) -> sparse.csr_array:
               [dg(fy @ ez), None, dg(fy @ -ex)],
               [dg(fz @ -ey), dg(fz @ ex), None]]
     return (sparse.block_array(blocks) @ dg(numpy.concatenate(dxb))).tocsr()
+
+
+def poynting_h_cross(h, dxes) -> sparse.csr_array:
+    """(H x) part of the Poynting vector (fdfd/operators.py:339-363): P @ e = -(E x H)
+    with E shifted one cell forward and both fields width-weighted."""
+    shape = _grid_shape(dxes[0])
+    fx, fy, fz = (shift_circ(a, shape, 1) for a in range(3))
+    dxa = [d.ravel(order='C') for d in numpy.meshgrid(*dxes[0], indexing='ij')]
+    dxb = [d.ravel(order='C') for d in numpy.meshgrid(*dxes[1], indexing='ij')]
+    dg = sparse.diags_array
+    hx, hy, hz = (dg(c * w) for c, w in zip(numpy.split(numpy.asarray(h), 3), dxb))
+    # each block is a width-weighted H component times a forward shift of E
+    blocks = [[None, -hz @ fx, hy @ fx],
+              [hz @ fy, None, -hx @ fy],
+              [-hy @ fz, hx @ fz, None]]
+    return (sparse.block_array(blocks) @ dg(numpy.concatenate(dxa))).tocsr()
 
 
 def shift_with_mirror(axis: int, shape: Sequence[int], shift_distance: int = 1) -> sparse.csr_array:

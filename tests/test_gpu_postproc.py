@@ -88,6 +88,21 @@ def test_poynting_cross_matches_oracle(shape):
     assert rel(s, (P @ h.conj()).real / 2) < 1e-13
 
 
+@pytest.mark.parametrize('shape', [(8, 9, 10), (12, 1, 9), (16, 12, 70)])
+def test_poynting_h_cross_matches_oracle(shape):
+    """poynting_h_cross(h, dxes) @ e against the oracle's sparse matrix, and with E
+    conjugated on the device as examples/fdfd1.py:123 uses it (with E.conj() on the host)."""
+    from meanas_amd.fdfd import functional
+    dxes, rng = _pml_grid(shape, 3)
+    n3 = 3 * int(numpy.prod(shape))
+    e, h = _rand(rng, n3), _rand(rng, n3)
+    P = functional.poynting_h_cross(h, dxes)
+    P_ref = ref.poynting_h_cross(h, dxes)
+    assert rel(P @ e, P_ref @ e) < 1e-13
+    assert rel(P.apply(e, conj_e=True), P_ref @ e.conj()) < 1e-13
+    numpy.testing.assert_array_equal(P.apply(e, conj_e=True), P @ e.conj())
+
+
 def test_field_layouts():
     """(3, nx, ny, nz) in -> same shape out; torch tensors stay on the device."""
     from meanas_amd.fdfd import functional

@@ -501,3 +501,16 @@ def test_m2j_plane_wave_closed_form():
     sp = ref.m2j(omega, dxes, numpy.full(3 * numpy.prod(shape), mu0)) @ ref.vec(m)
     numpy.testing.assert_allclose(sp, -ref.vec(got), rtol=1e-12, atol=1e-12)
     numpy.testing.assert_allclose(ref.m2j_functional(omega, dxes)(m), got * mu0, rtol=1e-12, atol=1e-12)
+
+
+def test_poynting_h_cross_is_minus_e_cross():
+    """poynting_h_cross(h) @ e = -(poynting_e_cross(e) @ h): both hold the same width-weighted
+    products, E shifted forward, with the roles of the two fields exchanged."""
+    rng = numpy.random.default_rng(21)
+    shape = (4, 3, 5)
+    n3 = 3 * int(numpy.prod(shape))
+    dxes = [[0.5 + rng.random(n) + 0.1j * rng.random(n) for n in shape] for _ in range(2)]
+    e = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    h = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
+    numpy.testing.assert_allclose(ref.poynting_h_cross(h, dxes) @ e, -(ref.poynting_e_cross(e, dxes) @ h),
+                                  rtol=1e-13, atol=1e-13)
