@@ -120,14 +120,27 @@ bool merged_fin_enabled() {
   return !(v && v[0] == '0');
 }
 
-int auto_slab(const mx_operator& op, int ty) {
-  // enough blocks to fill 256 CUs several times over, but deep slabs to amortise the
-  // two extra planes a slab reads; never more partials than one finishing block sums
+// Automatic slab depth: enough blocks to fill 256 CUs several times over, but deep slabs
+// to amortise the two extra planes a slab reads.  With `reduce` (kernels that leave
+// per-block partial sums) the block count must also stay within what one finishing
+// block sums; returns -1 when even a full-depth slab (one block per tile) exceeds it,
+// i.e. when the (y, z) cross-section alone has more than kMaxPartials tiles.
+int auto_slab(const mx_operator& op, int ty, bool reduce) {
   const long long tiles = (long long)((op.ny + ty - 1) / ty) * ((op.nz + 63) / 64);
   int slab = 16;
   while (slab > 4 && tiles * ((op.nx + slab - 1) / slab) < 2048) slab /= 2;
-  while (tiles * ((op.nx + slab - 1) / slab) > kMaxPartials) slab *= 2;
-  return std::min(slab, op.nx);
+  if (reduce)
+    while (slab < op.nx && tiles * ((op.nx + slab - 1) / slab) > kMaxPartials) slab *= 2;
+  slab = std::min(slab, op.nx);
+  if (reduce && tiles * ((op.nx + slab - 1) / slab) > kMaxPartials) return -1;
+  return slab;
+}
+
+std::string too_many_tiles(const mx_operator& op, int ty) {
+  const long long tiles = (long long)((op.ny + ty - 1) / ty) * ((op.nz + 63) / 64);
+  return "the " + std::to_string(op.ny) + " x " + std::to_string(op.nz) + " cross-section needs " +
+         std::to_string(tiles) + " apply tiles of " + std::to_string(ty) + " x 64 cells, more than the " +
+         std::to_string(kMaxPartials) + " per-system partial sums one finishing block reduces";
 }
 
 int apply_blocks(const mx_operator& op, int ty, int slab) {
@@ -245,7 +258,8 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     SysDesc d;
     std::memset(&d, 0, sizeof(d));
     d.op = y.op;
-    d.slab = y.slab > 0 ? std::min(y.slab, op.nx) : o.slab > 0 ? std::min(o.slab, op.nx) : auto_slab(op, ty);
+    d.slab = y.slab > 0 ? std::min(y.slab, op.nx) : o.slab > 0 ? std::min(o.slab, op.nx) : auto_slab(op, ty, true);
+    if (d.slab < 1) return fail("system " + std::to_string(s) + ": " + too_many_tiles(op, ty));
     d.nblk_apply = apply_blocks(op, ty, d.slab);
     d.n3 = 3 * ncell_of(op);
     d.nblk_vec = vec_blocks(d.n3);
@@ -444,7 +458,8 @@ mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, 
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
   d.op = 0;
-  d.slab = slab > 0 ? std::min<int>(slab, op->nx) : auto_slab(*op, ty);
+  // a plan's apply leaves no partial sums, so its block count has no reduction limit
+  d.slab = slab > 0 ? std::min<int>(slab, op->nx) : auto_slab(*op, ty, false);
   d.nblk_apply = apply_blocks(*op, ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   std::vector<BlockMap> nat;
@@ -519,7 +534,8 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   const int ty = kAutoTileRows;
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
-  d.slab = auto_slab(*op, ty);
+  d.slab = auto_slab(*op, ty, true);
+  if (d.slab < 1) return fail("mx_residual: " + too_many_tiles(*op, ty));
   d.nblk_apply = apply_blocks(*op, ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   d.nblk_vec = vec_blocks(d.n3);
@@ -944,7 +960,8 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   std::memset(&d, 0, sizeof(d));
   d.op = slot;   // one operator per slot
   d.slab = y->slab > 0 ? std::min<int>(y->slab, op->nx)
-         : S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty);
+         : S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty, true);
+  if (d.slab < 1) return fail(too_many_tiles(*op, S->ty));
   d.nblk_apply = apply_blocks(*op, S->ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   d.nblk_vec = vec_blocks(d.n3);

@@ -66,6 +66,39 @@ def _material(values, n: int, name: str):
     return numpy.ascontiguousarray(out), is_complex, aniso
 
 
+def operator_flags(shape, epsilon, mu=None, pec=None, pmc=None) -> int:
+    """The mx_operator flags DeviceOperator would set for these materials (host only)."""
+    n = int(numpy.prod(shape))
+    _, cplx, aniso = _material(epsilon, n, 'epsilon')
+    flags = (_lib.MX_EPS_COMPLEX if cplx else 0) | (_lib.MX_EPS_ANISO if aniso else 0)
+    if mu is not None:
+        _, cplx_m, aniso_m = _material(1 / numpy.asarray(mu), n, 'mu')
+        flags |= _lib.MX_MU | (_lib.MX_MU_COMPLEX if cplx_m else 0) | (_lib.MX_MU_ANISO if aniso_m else 0)
+    if any(m is not None and numpy.any(numpy.asarray(m) != 0) for m in (pec, pmc)):
+        flags |= _lib.MX_MASK
+    return flags
+
+
+def device_bytes(shape, flags: int, mode: int) -> int:
+    """HBM bytes one solve of this operator holds: the per-axis tables and the materials
+    in both precisions, the mask, b and x (complex128) plus the source upload, and the
+    solver workspace (mx_work_bytes, a host query)."""
+    nx, ny, nz = (int(v) for v in shape)
+    n = nx * ny * nz
+
+    def mat(cplx, aniso):
+        return (3 if aniso else 1) * n * (8 + 4) * (2 if cplx else 1)
+    total = (nx + ny + nz) * 4 * (16 + 8)
+    total += mat(flags & _lib.MX_EPS_COMPLEX, flags & _lib.MX_EPS_ANISO)
+    if flags & _lib.MX_MU:
+        total += mat(flags & _lib.MX_MU_COMPLEX, flags & _lib.MX_MU_ANISO)
+    if flags & _lib.MX_MASK:
+        total += n
+    total += 3 * 16 * 3 * n
+    desc = _lib.MxOperator(nx, ny, nz, flags, 0.0, 0.0, None, None, None, None, None, None, None)
+    return total + int(_lib.lib().mx_work_bytes(ctypes.byref(desc), mode))
+
+
 def _to_device(a: numpy.ndarray, dtype: torch.dtype, device) -> torch.Tensor:
     return torch.from_numpy(numpy.ascontiguousarray(a)).to(device=device, dtype=dtype)
 
@@ -226,6 +259,40 @@ class ApplyPlan:
             self.handle = None
 
 
+def project_masked(op: DeviceOperator, b: torch.Tensor) -> tuple[torch.Tensor, float]:
+    """Split off the part of b on PEC entries.
+
+    e_full has zero rows (and columns) on PEC entries (fdfd/operators.py:84-95), so b's
+    entries there can never be matched: A x = b is inconsistent, and COCG on it would mix
+    those constant entries into r^T r.  Returns (b with them zeroed, ||b on them||^2);
+    the solve of the projected system is the least-squares solution of the original,
+    whose residual norm is exactly that floor.  b itself is returned when nothing is
+    masked or b vanishes there.
+    """
+    if op.mask is None:
+        return b, 0.0
+    bits = op.mask.to(torch.int32)
+    pec = torch.cat([((bits >> c) & 1).bool() for c in range(3)])
+    floor2 = float(torch.sum(torch.abs(b[pec]) ** 2).item())
+    if floor2 == 0.0:
+        return b, 0.0
+    bp = b.clone()
+    bp[pec] = 0
+    return bp, floor2
+
+
+def mark_inconsistent(st: dict, floor2: float) -> dict:
+    """Stats of a solve whose right-hand side had a part on PEC entries (project_masked):
+    status 'inconsistent'; rel_residual becomes the returned iterate's residual relative to
+    the full ||b|| (the floor included); residual_floor is the floor's share of it."""
+    bn2 = st['bnorm'] ** 2 + floor2
+    free2 = (st['rel_residual'] * st['bnorm']) ** 2
+    out = dict(st, bnorm=bn2 ** 0.5, rel_residual=((free2 + floor2) / bn2) ** 0.5,
+               residual_floor=(floor2 / bn2) ** 0.5, solved_status=st['status'])
+    out['status'] = 'inconsistent'
+    return out
+
+
 def solve_batch(ops: Sequence[DeviceOperator], systems: Sequence[dict], mode: int = _lib.MX_MIXED,
                 options: dict | None = None, stream=None, fp64_fallback: bool = True) -> list[dict]:
     """Solve several systems in one batched launch sequence.
@@ -236,8 +303,20 @@ def solve_batch(ops: Sequence[DeviceOperator], systems: Sequence[dict], mode: in
     cannot resolve the operator, e.g. near-static omega with kappa ~ 1/eps32) continue
     in FP64 COCG from their FP64 iterate; their stats then carry fp64_fallback=True and
     the summed iteration counts.
+    A right-hand side with a part on PEC entries is solved without it (project_masked)
+    and reported with status 'inconsistent' (mark_inconsistent).
     """
+    floors = []
+    systems = list(systems)
+    for i, s in enumerate(systems):
+        bp, f2 = project_masked(ops[s['op']], s['b'])
+        floors.append(f2)
+        if f2 > 0:
+            systems[i] = dict(s, b=bp)
     stats = _solve_once(ops, systems, mode, options, stream)
+    if mode == _lib.MX_MIXED:
+        # every mixed-precision result says whether complex64 finished it on its own
+        stats = [dict(st, fp64_fallback=False, mixed_iterations=st['iterations']) for st in stats]
     if mode == _lib.MX_MIXED and fp64_fallback:
         redo = [i for i, st in enumerate(stats) if st['status'] in ('stalled', 'breakdown')]
         if redo:
@@ -253,7 +332,7 @@ def solve_batch(ops: Sequence[DeviceOperator], systems: Sequence[dict], mode: in
                           replacements=st['replacements'] + stats[i]['replacements'],
                           mixed_iterations=stats[i]['iterations'])
                 stats[i] = st
-    return stats
+    return [mark_inconsistent(st, f2) if f2 > 0 else st for st, f2 in zip(stats, floors)]
 
 
 def _solve_once(ops, systems, mode, options, stream) -> list[dict]:

@@ -1,16 +1,24 @@
 """Continuous-batching FDFD solve server on one GPU.
 
-Requests (the arguments of meanas.fdfd.solvers.generic, forward or adjoint) arrive over
-time.  The server keeps a session of slots on the device (`mx_session_*`, C ABI): every
-step runs `chunk` COCG iterations on all active slots in one set of launches; finished
-slots are read out and retired and waiting requests are admitted as soon as a slot and
-enough HBM are free, so new work joins while other solves are in flight.  A request's
-result does not depend on what else is in the session: each system's block partition
-and reduction order depend on that system alone.
+Requests (the arguments of meanas.fdfd.solvers.generic, forward or adjoint; reference
+meanas/fdfd/solvers.py:66-141) arrive over time.  The server keeps a session of slots on
+the device (`mx_session_*`, C ABI): every step runs `chunk` COCG iterations on all
+active slots in one set of launches; finished slots are read out and retired and
+waiting requests are admitted as soon as a slot and enough HBM are free, so new work
+joins while other solves are in flight.  A request's result does not depend on what
+else is in the session: each system's block partition and reduction order depend on
+that system alone.
 
-Admission is by HBM footprint: operator tables and materials, b and x (complex128) and
-the solver workspace (`mx_work_bytes`).  Host preparation of arrived requests (PML
+Admission is by HBM footprint, worked out on the host before anything is allocated:
+the operator tables and materials (complex or anisotropic eps, 1/mu, the PEC/PMC mask),
+b, x and the source upload, and the solver workspace (`mx_work_bytes`).  A request's
+footprint is reserved when its host preparation starts, and that preparation (PML
 tables, material compaction, uploads) overlaps with the iterations already queued.
+
+In mixed precision (complex64 iterations, FP64 residual replacement) a request whose
+complex64 iterations stall or break down continues in an FP64 session from its best
+FP64 iterate, as `solve_batch` does; its result carries fp64_fallback and the summed
+iteration counts.
 """
 from __future__ import annotations
 
@@ -23,7 +31,8 @@ import numpy
 import torch
 
 from . import _lib
-from .fdfd.device import DeviceOperator, torch_stream_handle
+from .fdfd.device import (DeviceOperator, device_bytes, mark_inconsistent, operator_flags, project_masked,
+                          torch_stream_handle)
 
 
 @dataclass
@@ -61,26 +70,29 @@ class Result:
 @dataclass
 class _Slot:
     req: Request
-    op: DeviceOperator
-    b: torch.Tensor
-    x: torch.Tensor
-    work: torch.Tensor
+    op: DeviceOperator | None
+    b: torch.Tensor | None
+    x: torch.Tensor | None
+    work: torch.Tensor | None
     footprint: int
+    mode: int
     admitted: float = 0.0
+    prior: dict | None = None      # stats of a mixed-precision phase this FP64 slot continues
     extra: dict = field(default_factory=dict)
 
 
-def footprint_bytes(shape, mode: int) -> int:
-    """HBM bytes one admitted request holds (approximate for the operator tables)."""
-    n3 = 3 * int(numpy.prod(shape))
-    vec = 16 * n3                                    # b and x, complex128
-    work = 4 * (8 if mode == _lib.MX_MIXED else 16) * n3 + 16 * n3
-    mats = 12 * n3                                   # eps (f32 + f64), up to anisotropic
-    return 2 * vec + work + mats
+def request_shape(req: Request) -> tuple[int, int, int]:
+    return tuple(int(numpy.size(d)) for d in req.dxes[0])
+
+
+def footprint_bytes(req: Request, mode: int) -> int:
+    """HBM bytes one admitted request holds, from its shape and material formats (host only)."""
+    shape = request_shape(req)
+    return device_bytes(shape, operator_flags(shape, req.epsilon, req.mu, req.pec, req.pmc), mode)
 
 
 class Server:
-    """Continuous-batching solver over one `mx_session`."""
+    """Continuous-batching solver over `mx_session`s (one per precision in use)."""
 
     def __init__(self, capacity: int = 8, mode: int = _lib.MX_FP64, chunk: int = 16,
                  hbm_budget: int | None = None, device: str = 'cuda', options: dict | None = None):
@@ -96,69 +108,127 @@ class Server:
                  stall_iters=0, stall_limit=0, form='auto')
         if options:
             o.update(options)
-        self.opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'],
-                                   o['delta'], o['stall_iters'], o['stall_limit'], _lib.FORMS[o['form']])
-        L = _lib.lib()
-        self.handle = L.mx_session_create(capacity, ctypes.byref(self.opts))
-        if not self.handle:
-            raise _lib.MxError('mx_session_create failed: ' + L.mx_last_error().decode(errors='replace'))
-        self.slots: dict[int, _Slot] = {}
-        self.used = 0
+        self._o = o
+        self.sessions: dict[int, int] = {}
+        self._session(mode)
+        self.slots: dict[tuple[int, int], _Slot] = {}
+        self.used = 0                   # bytes reserved by prepared and admitted requests
+
+    def _session(self, mode: int):
+        if mode not in self.sessions:
+            o = self._o
+            form = o['form'] if (mode == _lib.MX_FP64 or o['form'] != 'fused') else 'auto'
+            opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'],
+                                  o['delta'], o['stall_iters'], o['stall_limit'], _lib.FORMS[form])
+            L = _lib.lib()
+            h = L.mx_session_create(self.capacity, ctypes.byref(opts))
+            if not h:
+                raise _lib.MxError('mx_session_create failed: ' + L.mx_last_error().decode(errors='replace'))
+            self.sessions[mode] = h
+        return self.sessions[mode]
+
+    @property
+    def handle(self):
+        return self.sessions.get(self.mode)
 
     def close(self):
-        if getattr(self, 'handle', None):
-            _lib.lib().mx_session_destroy(self.handle)
-            self.handle = None
+        for h in getattr(self, 'sessions', {}).values():
+            _lib.lib().mx_session_destroy(h)
+        self.sessions = {}
 
     def __del__(self):
         self.close()
 
-    # ---- request preparation (host + uploads on torch's stream) ---------------------
-    def prepare(self, req: Request) -> _Slot:
-        op = DeviceOperator(req.omega, req.dxes, req.epsilon, mu=req.mu, pec=req.pec, pmc=req.pmc,
-                            device=self.device)
-        J = torch.from_numpy(numpy.ascontiguousarray(req.J, dtype=numpy.complex128)).to(self.device)
-        which = _lib.MX_MAP_RHS_ADJ if req.adjoint else _lib.MX_MAP_RHS
-        b = op.map(which, J, coef=-1j * complex(req.omega))
-        x = torch.zeros_like(b)
-        work = torch.empty(op.work_bytes(self.mode), dtype=torch.uint8, device=self.device)
-        return _Slot(req, op, b, x, work, footprint_bytes(op.shape, self.mode))
+    # ---- admission ---------------------------------------------------------------
+    def active(self, mode: int) -> int:
+        return sum(1 for (m, _) in self.slots if m == mode)
 
-    def fits(self, shape) -> bool:
-        return (len(self.slots) < self.capacity and
-                self.used + footprint_bytes(shape, self.mode) <= self.hbm_budget)
+    def fits(self, nbytes: int) -> bool:
+        return self.used + nbytes <= self.hbm_budget
+
+    def prepare(self, req: Request, footprint: int | None = None) -> _Slot:
+        """Reserve the request's footprint, then build its operator and b on the device.
+        Raises MxError (nothing allocated, nothing reserved) when it does not fit."""
+        fp = footprint_bytes(req, self.mode) if footprint is None else footprint
+        if not self.fits(fp):
+            raise _lib.MxError(f'request of shape {request_shape(req)} needs {fp} bytes of HBM; '
+                               f'{self.hbm_budget - self.used} of the {self.hbm_budget}-byte budget are free')
+        self.used += fp
+        try:
+            op = DeviceOperator(req.omega, req.dxes, req.epsilon, mu=req.mu, pec=req.pec, pmc=req.pmc,
+                                device=self.device)
+            J = torch.from_numpy(numpy.ascontiguousarray(req.J, dtype=numpy.complex128)).to(self.device)
+            which = _lib.MX_MAP_RHS_ADJ if req.adjoint else _lib.MX_MAP_RHS
+            b = op.map(which, J, coef=-1j * complex(req.omega))
+            del J
+            b, floor2 = project_masked(op, b)      # a source inside PEC cells: least squares
+            x = torch.zeros_like(b)
+            work = torch.empty(op.work_bytes(self.mode), dtype=torch.uint8, device=self.device)
+        except Exception:
+            self.used -= fp
+            raise
+        return _Slot(req, op, b, x, work, fp, self.mode, extra=dict(floor2=floor2))
 
     def admit(self, slot: _Slot, now: float) -> int:
         req = slot.req
         maxiter = req.maxiter if req.maxiter is not None else 10 * slot.op.n3
+        if slot.prior:
+            maxiter = max(1, maxiter - slot.prior['iterations'])
         from .fdfd.autotune import tuned_slab
-        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), tuned_slab(slot.op, self.mode), 0, req.rtol, req.atol,
+        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), tuned_slab(slot.op, slot.mode), 0, req.rtol, req.atol,
                              slot.b.data_ptr(), slot.x.data_ptr(), slot.work.data_ptr())
-        idx = _lib.lib().mx_session_admit(self.handle, ctypes.byref(slot.op.desc), ctypes.byref(sysd),
-                                          torch_stream_handle())
+        h = self._session(slot.mode)
+        idx = _lib.lib().mx_session_admit(h, ctypes.byref(slot.op.desc), ctypes.byref(sysd), torch_stream_handle())
         if idx < 0:
             raise _lib.MxError('mx_session_admit failed: ' + _lib.lib().mx_last_error().decode(errors='replace'))
-        slot.admitted = now
-        self.slots[idx] = slot
-        self.used += slot.footprint
+        if not slot.prior:
+            slot.admitted = now
+        self.slots[(slot.mode, idx)] = slot
         return idx
 
     def step(self) -> None:
-        _lib.check(_lib.lib().mx_session_step(self.handle, self.chunk, None), 'mx_session_step')
+        for mode, h in self.sessions.items():
+            if self.active(mode):
+                _lib.check(_lib.lib().mx_session_step(h, self.chunk, None), 'mx_session_step')
 
-    def poll(self):
+    def poll(self, mode: int):
         stats = (_lib.MxStats * self.capacity)()
         done = (ctypes.c_int32 * self.capacity)()
-        _lib.check(_lib.lib().mx_session_poll(self.handle, stats, done, torch_stream_handle()), 'mx_session_poll')
+        _lib.check(_lib.lib().mx_session_poll(self.sessions[mode], stats, done, torch_stream_handle()),
+                   'mx_session_poll')
         return stats, done
 
-    def finish(self, idx: int, stats: _lib.MxStats, now: float) -> Result:
-        slot = self.slots.pop(idx)
+    def finish(self, key, stats: _lib.MxStats, now: float):
+        """Retire a finished slot: a Result, or (mixed precision that stalled) the slot to
+        continue in FP64 from its best iterate (its workspace traded for an FP64 one)."""
+        mode, idx = key
+        slot = self.slots.pop(key)
+        _lib.check(_lib.lib().mx_session_retire(self.sessions[mode], idx), 'mx_session_retire')
+        st = stats.as_dict()
+        if mode == _lib.MX_MIXED:
+            st.update(fp64_fallback=False, mixed_iterations=st['iterations'])
+            if st['status'] in ('stalled', 'breakdown'):
+                slot.work = None
+                self.used -= slot.footprint
+                slot.footprint = footprint_bytes(slot.req, _lib.MX_FP64)
+                slot.mode = _lib.MX_FP64
+                slot.prior = st
+                return slot
+        if slot.prior:
+            p = slot.prior
+            st = dict(st, fp64_fallback=True, iterations=st['iterations'] + p['iterations'],
+                      replacements=st['replacements'] + p['replacements'], mixed_iterations=p['iterations'])
+        if slot.extra.get('floor2', 0.0) > 0:
+            st = mark_inconsistent(st, slot.extra['floor2'])
         which = _lib.MX_MAP_E_ADJ if slot.req.adjoint else _lib.MX_MAP_E
         E = slot.op.map(which, slot.x).cpu().numpy()
-        _lib.check(_lib.lib().mx_session_retire(self.handle, idx), 'mx_session_retire')
         self.used -= slot.footprint
-        return Result(slot.req.tag, E, stats.as_dict(), slot.req.arrival, slot.admitted, now, slot.op.shape)
+        return Result(slot.req.tag, E, st, slot.req.arrival, slot.admitted, now, slot.op.shape)
+
+    def _continue(self, slot: _Slot, now: float) -> None:
+        self.used += slot.footprint
+        slot.work = torch.empty(slot.op.work_bytes(_lib.MX_FP64), dtype=torch.uint8, device=self.device)
+        self.admit(slot, now)
 
     # ---- the request loop ------------------------------------------------------------
     def run(self, requests, realtime: bool = True) -> list[Result]:
@@ -166,30 +236,50 @@ class Server:
         clock from the start of the run; otherwise every request is available at once."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         ready: collections.deque[_Slot] = collections.deque()
+        cont: collections.deque[_Slot] = collections.deque()      # mixed -> FP64 continuations
         results: list[Result] = []
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0)
-        while pending or ready or self.slots:
+        next_fp = None
+        while pending or ready or cont or self.slots:
             now = clock()
             if self.slots:
                 self.step()                       # queued: overlaps the host work below
-            # prepare arrived requests while the GPU iterates
+            while cont and self.active(_lib.MX_FP64) < self.capacity and self.fits(cont[0].footprint):
+                self._continue(cont.popleft(), clock())
+            # prepare arrived requests while the GPU iterates; the footprint is checked
+            # and reserved before anything is allocated for the request
             while pending and (not realtime or pending[0].arrival <= now) and \
-                    len(ready) + len(self.slots) < self.capacity:
-                ready.append(self.prepare(pending.popleft()))
-            if self.slots:
-                stats, done = self.poll()
+                    len(ready) + self.active(self.mode) < self.capacity:
+                if next_fp is None:
+                    next_fp = footprint_bytes(pending[0], self.mode)
+                if not self.fits(next_fp):
+                    break
+                ready.append(self.prepare(pending.popleft(), next_fp))
+                next_fp = None
+            for mode in list(self.sessions):
+                if not self.active(mode):
+                    continue
+                stats, done = self.poll(mode)
                 now = clock()
-                for i in [i for i in list(self.slots) if done[i] == 1]:
-                    results.append(self.finish(i, stats[i], now))
-            while ready and self.fits(ready[0].op.shape):
+                for key in [k for k in list(self.slots) if k[0] == mode and done[k[1]] == 1]:
+                    out = self.finish(key, stats[key[1]], now)
+                    if isinstance(out, Result):
+                        results.append(out)
+                    else:
+                        cont.append(out)
+            while ready and self.active(self.mode) < self.capacity:
                 self.admit(ready.popleft(), clock())
-            if not self.slots and not ready and pending and realtime:
-                wait = pending[0].arrival - clock()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-            if ready and not self.slots and not self.fits(ready[0].op.shape):
-                raise _lib.MxError(f'request of shape {ready[0].op.shape} does not fit the HBM budget')
+            if not self.slots and not ready and not cont and pending:
+                if next_fp is not None and not self.fits(next_fp) and (not realtime or pending[0].arrival <= clock()):
+                    raise _lib.MxError(f'request of shape {request_shape(pending[0])} needs {next_fp} bytes of '
+                                       f'HBM, more than the {self.hbm_budget}-byte budget')
+                if realtime:
+                    wait = pending[0].arrival - clock()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+            if not self.slots and not ready and cont and not self.fits(cont[0].footprint):
+                raise _lib.MxError(f'FP64 continuation of shape {cont[0].op.shape} does not fit the HBM budget')
         return results
 
 

@@ -47,6 +47,8 @@ def test_multi_rhs_batch(method):
     st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=1e-9) for b, x in zip(bs, xs)], mode=mode)
     for k, (J, x) in enumerate(zip(Js, xs)):
         assert st[k]['status'] == 'converged', st[k]
+        if mode == _lib.MX_MIXED:   # finished by the complex64 iterations themselves
+            assert st[k]['fp64_fallback'] is False, st[k]
         E = op.map(_lib.MX_MAP_E, x).cpu().numpy()
         assert physical_residual(omega, dxes, eps, J, E) < 1e-8
     # the first system alone gives the same bits

@@ -55,6 +55,26 @@ def test_work_bytes_is_a_pure_host_query():
     assert _lib.lib().mx_work_bytes(ctypes.byref(op), 7) == 0
 
 
+@pytest.mark.parametrize('shape', [(1, 2048, 2048), (64, 1024, 1280)])
+def test_oversized_cross_section_is_refused_on_the_host(shape):
+    """A (y, z) cross-section with more apply tiles than one finishing block reduces is
+    refused with a clear error by every entry point whose kernels leave partial sums,
+    before any device call (the pointers below are never dereferenced)."""
+    from meanas_amd import _lib
+    L = _lib.lib()
+    fake = 0x1000
+    op = _lib.MxOperator(*shape, 0, 1.0, 0.0, fake, fake, fake, fake, None, None, None)
+    assert L.mx_work_bytes(ctypes.byref(op), _lib.MX_FP64) > 0
+    norm = ctypes.c_double(0.0)
+    assert L.mx_residual(ctypes.byref(op), fake, fake, fake, ctypes.byref(norm), None) != 0
+    assert b'apply tiles' in L.mx_last_error(), L.mx_last_error()
+    sysd = _lib.MxSystem(0, 10, 0, 0, 1e-8, 0.0, fake, fake, fake)
+    opts = _lib.MxOptions(_lib.MX_FP64, 0, 0, 0, 0, 1, 0.0, 0, 0, 0, 0)
+    stats = (_lib.MxStats * 1)()
+    assert L.mx_solve(ctypes.byref(op), 1, ctypes.byref(sysd), 1, ctypes.byref(opts), stats, None) != 0
+    assert b'apply tiles' in L.mx_last_error(), L.mx_last_error()
+
+
 def test_conducting_boundary_checks_arguments_on_the_host():
     """mx_fdtd_conducting_boundary refuses bad arguments before any device call (each
     call below fails a check; the buffer is never handed to a kernel)."""
