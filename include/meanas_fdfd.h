@@ -143,6 +143,7 @@ void mx_plan_destroy(mx_plan* plan);
  * The residual the reference's logging callback prints (fdfd/solvers.py:46-48). */
 int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream);
 
+
 /* Diagonal maps of generic() (fdfd/solvers.py:115,120-139), complex128 in/out:
  *   MX_MAP_RHS      out = p * (c * in)              b = Pl (-i w J): c = -i w
  *   MX_MAP_RHS_ADJ  out = conj(c * in) / p          conj of Pr^H (-i w J) (adjoint)

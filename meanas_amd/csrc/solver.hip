@@ -201,6 +201,35 @@ std::vector<BlockMap> xcd_order(const std::vector<BlockMap>& natural) {
 }
 
 // ---- a batch of systems with device-resident tables ---------------------------------
+// Launch groups.  Systems are launched in groups whose hot data -- the vectors the apply
+// and the update pass between each other (p, q) and the materials -- fits the 256 MB
+// Infinity Cache (MALL); a captured chunk runs all of its iterations on one group before
+// the next, so each group's p, q and eps stay cache-resident from kernel to kernel as
+// in a single solve (x_lo and r are streamed past the cache by the update).  A group of
+// many small systems also fills the GPU that one of them would leave idle.  A system's
+// block partition does not depend on its group, so results do not either.
+struct Group { int a_off, na, v_off, nv, s_off, ns; };
+
+// MALL bytes a launch group may fill (MEANAS_AMD_CACHE_MB overrides; 0: one group)
+size_t cache_budget() {
+  static const size_t b = [] {
+    const char* v = getenv("MEANAS_AMD_CACHE_MB");
+    return (size_t)(v ? atof(v) : 224.0) * (1u << 20);
+  }();
+  return b;
+}
+
+// bytes of one system's p and q, and of its operator's materials, in the inner precision
+size_t vec_hot_bytes(const mx_operator& op, int mode) { return 2 * (size_t)(3 * ncell_of(op)) * elem_bytes(mode); }
+size_t mat_hot_bytes(const mx_operator& op, int mode) {
+  const size_t n = (size_t)ncell_of(op), real = mode == MX_MIXED ? 4 : 8;
+  auto mat = [&](bool cplx, bool aniso) { return (aniso ? 3 : 1) * n * real * (cplx ? 2 : 1); };
+  size_t b = mat(op.flags & MX_EPS_COMPLEX, op.flags & MX_EPS_ANISO);
+  if (op.flags & MX_MU) b += mat(op.flags & MX_MU_COMPLEX, op.flags & MX_MU_ANISO);
+  if (op.flags & MX_MASK) b += n;
+  return b;
+}
+
 struct Batch {
   int mode = MX_MIXED;
   int ty = 4;
@@ -214,7 +243,15 @@ struct Batch {
   BlockMap* d_bmap_vec = nullptr;
   int* d_syslist = nullptr;   // systems the finishing kernels visit (one block each)
   int nsl = 0;
+  // the whole (all groups) maps; d_bmap_apply etc. point at one group while it is enqueued
+  BlockMap* bm_apply0 = nullptr;
+  BlockMap* bm_vec0 = nullptr;
+  int* syslist0 = nullptr;
+  int nb_apply0 = 0, nb_vec0 = 0, nsl0 = 0;
+  std::vector<Group> groups;
   std::vector<SysDesc> h_sys;
+  std::vector<int> h_op;        // operator index of each system
+  const mx_operator* ops = nullptr;
   SysState* h_state[2] = {nullptr, nullptr};   // pinned, double-buffered polls
   hipStream_t stream = nullptr;                  // library-owned capture stream
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -224,6 +261,17 @@ struct Batch {
   bool classic_launch = true;   // enqueue the classic-form direction update every iteration
   bool merged_fin = false;      // every system stays fused: one finishing launch per iteration
   bool simple = false;          // every operator has simple materials (no mask, no mu, real isotropic eps)
+
+  void use_group(const Group& g) {
+    d_bmap_apply = bm_apply0 + g.a_off; nb_apply = g.na;
+    d_bmap_vec = bm_vec0 + g.v_off; nb_vec = g.nv;
+    d_syslist = syslist0 + g.s_off; nsl = g.ns;
+  }
+  void use_all() {
+    d_bmap_apply = bm_apply0; nb_apply = nb_apply0;
+    d_bmap_vec = bm_vec0; nb_vec = nb_vec0;
+    d_syslist = syslist0; nsl = nsl0;
+  }
 
   ~Batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -235,19 +283,68 @@ struct Batch {
   }
 };
 
+// Lay out the block maps and system lists of the `active` systems as consecutive launch
+// groups (see Group) and copy them to the device (synchronous; the caller has drained
+// the stream).  Each group's maps are XCD-ordered on their own.
+int layout_groups(Batch& B, const std::vector<char>& active) {
+  const size_t budget = cache_budget();
+  std::vector<BlockMap> apply, vec;
+  std::vector<int> sl;
+  B.groups.clear();
+  std::vector<BlockMap> ga, gv;
+  std::vector<int> gs, gops;
+  size_t bytes = 0;
+  auto close = [&]() {
+    if (gs.empty()) return;
+    Group g;
+    g.a_off = (int)apply.size(); g.v_off = (int)vec.size(); g.s_off = (int)sl.size();
+    const std::vector<BlockMap> oa = xcd_order(ga), ov = xcd_order(gv);
+    apply.insert(apply.end(), oa.begin(), oa.end());
+    vec.insert(vec.end(), ov.begin(), ov.end());
+    sl.insert(sl.end(), gs.begin(), gs.end());
+    g.na = (int)oa.size(); g.nv = (int)ov.size(); g.ns = (int)gs.size();
+    B.groups.push_back(g);
+    ga.clear(); gv.clear(); gs.clear(); gops.clear();
+    bytes = 0;
+  };
+  for (int s = 0; s < B.nsys; ++s) {
+    if (!active[s]) continue;
+    const mx_operator& op = B.ops[B.h_op[s]];
+    const bool new_op = std::find(gops.begin(), gops.end(), B.h_op[s]) == gops.end();
+    const size_t add = vec_hot_bytes(op, B.mode) + (new_op ? mat_hot_bytes(op, B.mode) : 0);
+    if (budget > 0 && !gs.empty() && bytes + add > budget) close();
+    if (std::find(gops.begin(), gops.end(), B.h_op[s]) == gops.end()) gops.push_back(B.h_op[s]);
+    bytes += add;
+    gs.push_back(s);
+    for (int b = 0; b < B.h_sys[s].nblk_apply; ++b) ga.push_back({s, b});
+    for (int b = 0; b < B.h_sys[s].nblk_vec; ++b) gv.push_back({s, b});
+  }
+  close();
+  B.nb_apply0 = (int)apply.size();
+  B.nb_vec0 = (int)vec.size();
+  B.nsl0 = (int)sl.size();
+  if (!apply.empty()) MX_TRY(hipMemcpy(B.bm_apply0, apply.data(), sizeof(BlockMap) * apply.size(), hipMemcpyHostToDevice));
+  if (!vec.empty()) MX_TRY(hipMemcpy(B.bm_vec0, vec.data(), sizeof(BlockMap) * vec.size(), hipMemcpyHostToDevice));
+  if (!sl.empty()) MX_TRY(hipMemcpy(B.syslist0, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
+  B.use_all();
+  return 0;
+}
+
 int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys, int nsys, const mx_options& o) {
   std::string why;
   B.mode = o.mode;
   B.nops = nops;
   B.nsys = nsys;
+  B.ops = ops;
   const int ty = o.tile_rows ? o.tile_rows : kAutoTileRows;
   if (ty != 2 && ty != 4 && ty != 8) return fail("tile_rows must be 2, 4 or 8");
   B.ty = ty;
   for (int i = 0; i < nops; ++i)
     if (!valid_op(ops[i], o.mode, &why)) return fail("operator " + std::to_string(i) + ": " + why);
 
-  std::vector<BlockMap> nat_apply, nat_vec;
+  size_t total_apply = 0, total_vec = 0;
   B.h_sys.resize(nsys);
+  B.h_op.resize(nsys);
   for (int s = 0; s < nsys; ++s) {
     const mx_system& y = sys[s];
     if (y.op < 0 || y.op >= nops) return fail("system " + std::to_string(s) + ": bad operator index");
@@ -277,32 +374,25 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.partial = reinterpret_cast<double*>(wk + w.off_partial);
     d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
     B.h_sys[s] = d;
-    for (int b = 0; b < d.nblk_apply; ++b) nat_apply.push_back({s, b});
-    for (int b = 0; b < d.nblk_vec; ++b) nat_vec.push_back({s, b});
+    B.h_op[s] = y.op;
+    total_apply += d.nblk_apply;
+    total_vec += d.nblk_vec;
   }
-  const std::vector<BlockMap> bm_apply = xcd_order(nat_apply);
-  const std::vector<BlockMap> bm_vec = xcd_order(nat_vec);
-  B.nb_apply = (int)bm_apply.size();
-  B.nb_vec = (int)bm_vec.size();
 
   const size_t s_ops32 = align_up(sizeof(OpView<float>) * nops);
   const size_t s_ops64 = align_up(sizeof(OpView<double>) * nops);
   const size_t s_sys = align_up(sizeof(SysDesc) * nsys);
-  const size_t s_ba = align_up(sizeof(BlockMap) * bm_apply.size());
-  const size_t s_bv = align_up(sizeof(BlockMap) * bm_vec.size());
+  const size_t s_ba = align_up(sizeof(BlockMap) * total_apply);
+  const size_t s_bv = align_up(sizeof(BlockMap) * total_vec);
   const size_t s_sl = align_up(sizeof(int) * nsys);
   MX_TRY(hipMalloc(&B.d_mem, s_ops32 + s_ops64 + s_sys + s_ba + s_bv + s_sl));
   char* p = B.d_mem;
   B.d_ops32 = reinterpret_cast<OpView<float>*>(p); p += s_ops32;
   B.d_ops64 = reinterpret_cast<OpView<double>*>(p); p += s_ops64;
   B.d_sys = reinterpret_cast<SysDesc*>(p); p += s_sys;
-  B.d_bmap_apply = reinterpret_cast<BlockMap*>(p); p += s_ba;
-  B.d_bmap_vec = reinterpret_cast<BlockMap*>(p); p += s_bv;
-  B.d_syslist = reinterpret_cast<int*>(p);
-  B.nsl = nsys;
-  std::vector<int> sl(nsys);
-  for (int i = 0; i < nsys; ++i) sl[i] = i;
-  MX_TRY(hipMemcpy(B.d_syslist, sl.data(), sizeof(int) * nsys, hipMemcpyHostToDevice));
+  B.bm_apply0 = reinterpret_cast<BlockMap*>(p); p += s_ba;
+  B.bm_vec0 = reinterpret_cast<BlockMap*>(p); p += s_bv;
+  B.syslist0 = reinterpret_cast<int*>(p);
 
   std::vector<OpView<float>> v32(nops);
   std::vector<OpView<double>> v64(nops);
@@ -316,8 +406,7 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
   MX_TRY(hipMemcpy(B.d_ops32, v32.data(), sizeof(OpView<float>) * nops, hipMemcpyHostToDevice));
   MX_TRY(hipMemcpy(B.d_ops64, v64.data(), sizeof(OpView<double>) * nops, hipMemcpyHostToDevice));
   MX_TRY(hipMemcpy(B.d_sys, B.h_sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(B.d_bmap_apply, bm_apply.data(), sizeof(BlockMap) * bm_apply.size(), hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(B.d_bmap_vec, bm_vec.data(), sizeof(BlockMap) * bm_vec.size(), hipMemcpyHostToDevice));
+  if (layout_groups(B, std::vector<char>(nsys, 1))) return -1;
 
   for (auto& h : B.h_state) MX_TRY(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(SysState) * nsys, hipHostMallocDefault));
   MX_TRY(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
@@ -365,7 +454,7 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
   return 0;
 }
 
-int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
+int enqueue_group_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
   for (int it = 0; it < iters; ++it) {
     if (enqueue_iteration(B, s)) return -1;
     if ((it + 1) % check_every == 0 || it + 1 == iters) {
@@ -376,6 +465,20 @@ int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
     }
   }
   return 0;
+}
+
+// `iters` iterations of every launch group, one group after the other (a batch view
+// without groups -- sessions -- is one group)
+int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
+  if (B.groups.empty()) return enqueue_group_chunk(B, s, iters, check_every);
+  int rc = 0;
+  for (const Group& g : B.groups) {
+    B.use_group(g);
+    rc = enqueue_group_chunk(B, s, iters, check_every);
+    if (rc) break;
+  }
+  B.use_all();
+  return rc;
 }
 
 // A finished system that did not converge returns its best iterate (restored by
@@ -526,53 +629,86 @@ int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const v
   return 0;
 }
 
-int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream) {
-  if (!op || !b || !x || !r) return fail("null argument");
+}  // extern "C"
+
+namespace {
+
+// Device memory of one FP64 residual r = b - A x: the operator and system descriptors,
+// the system state, the per-block partial sums (3 per apply block), the block map and a
+// one-entry system list, each aligned.
+struct ResidLayout { size_t s_op, s_sys, s_st, s_part, s_bm, total; };
+ResidLayout resid_layout(int nblk) {
+  ResidLayout L;
+  L.s_op = align_up(sizeof(OpView<double>));
+  L.s_sys = align_up(sizeof(SysDesc));
+  L.s_st = align_up(sizeof(SysState));
+  L.s_part = align_up(sizeof(double) * 3 * (size_t)nblk);
+  L.s_bm = align_up(sizeof(BlockMap) * (size_t)nblk);
+  L.total = L.s_op + L.s_sys + L.s_st + L.s_part + L.s_bm + align_up(sizeof(int));
+  return L;
+}
+
+// Checks and the tiling of a residual; fills d (vectors, tiling) or fails.
+int resid_setup(const mx_operator* op, const void* b, const void* x, void* r, const char* who, SysDesc* d) {
+  if (!op || !b || !x || !r) return fail(std::string(who) + ": null argument");
   std::string why;
-  if (!valid_op(*op, MX_FP64, &why)) return fail(why);
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  const int ty = kAutoTileRows;
-  SysDesc d;
-  std::memset(&d, 0, sizeof(d));
-  d.slab = auto_slab(*op, ty, true);
-  if (d.slab < 1) return fail("mx_residual: " + too_many_tiles(*op, ty));
-  d.nblk_apply = apply_blocks(*op, ty, d.slab);
-  d.n3 = 3 * ncell_of(*op);
-  d.nblk_vec = vec_blocks(d.n3);
-  d.vec[VS_X] = const_cast<void*>(x);
-  d.vec[VS_B] = const_cast<void*>(b);
-  d.vec[VS_R] = r;
-  std::vector<BlockMap> bm;
-  for (int i = 0; i < d.nblk_apply; ++i) bm.push_back({0, i});
-  const size_t s_op = align_up(sizeof(OpView<double>));
-  const size_t s_sys = align_up(sizeof(SysDesc));
-  const size_t s_st = align_up(sizeof(SysState));
-  const size_t s_part = align_up(sizeof(double) * 4 * (size_t)d.nblk_apply);
-  const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
-  char* mem = nullptr;
-  MX_TRY(hipMalloc(&mem, s_op + s_sys + s_st + s_part + s_bm + align_up(sizeof(int))));
-  int* d_one = reinterpret_cast<int*>(mem + s_op + s_sys + s_st + s_part + s_bm);
-  d.st = reinterpret_cast<SysState*>(mem + s_op + s_sys);
-  d.partial = reinterpret_cast<double*>(mem + s_op + s_sys + s_st);
+  if (!valid_op(*op, MX_FP64, &why)) return fail(std::string(who) + ": " + why);
+  std::memset(d, 0, sizeof(*d));
+  d->slab = auto_slab(*op, kAutoTileRows, true);
+  if (d->slab < 1) return fail(std::string(who) + ": " + too_many_tiles(*op, kAutoTileRows));
+  d->nblk_apply = apply_blocks(*op, kAutoTileRows, d->slab);
+  d->n3 = 3 * ncell_of(*op);
+  d->nblk_vec = vec_blocks(d->n3);
+  d->vec[VS_X] = const_cast<void*>(x);
+  d->vec[VS_B] = const_cast<void*>(b);
+  d->vec[VS_R] = r;
+  return 0;
+}
+
+// r = b - A x and ||r||^2 with all device-side state in `mem` (resid_layout bytes):
+// tables staged in stream order, the residual kernel and its finishing kernel, then the
+// state read back (the call returns when the stream has reached that point).
+int resid_run(const mx_operator& op, SysDesc d, char* mem, double* norm2_out, hipStream_t s, const char* who) {
+  const ResidLayout L = resid_layout(d.nblk_apply);
+  char* p_sys = mem + L.s_op;
+  d.st = reinterpret_cast<SysState*>(p_sys + L.s_sys);
+  d.partial = reinterpret_cast<double*>(p_sys + L.s_sys + L.s_st);
+  BlockMap* bmd = reinterpret_cast<BlockMap*>(p_sys + L.s_sys + L.s_st + L.s_part);
+  int* d_one = reinterpret_cast<int*>(reinterpret_cast<char*>(bmd) + L.s_bm);
+  std::vector<BlockMap> bm((size_t)d.nblk_apply);
+  for (int i = 0; i < d.nblk_apply; ++i) bm[i] = {0, i};
   SysState st;
   std::memset(&st, 0, sizeof(st));
   st.flags = SF_PAUSE | SF_RESTART;   // RESTART: no search direction to carry over
   st.stop2 = -1.0;
-  OpView<double> v = view_of<double>(*op);
-  int rc = 0;
-  hipError_t e = hipMemcpy(mem, &v, sizeof(v), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(d.st, &st, sizeof(st), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(mem + s_op + s_sys + s_st + s_part, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
+  const OpView<double> v = view_of<double>(op);
   const int zero_idx = 0;
-  if (e == hipSuccess) e = hipMemcpy(d_one, &zero_idx, sizeof(int), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_apply_resid_r64(ty, op_simple(*op), reinterpret_cast<const OpView<double>*>(mem), reinterpret_cast<const SysDesc*>(mem + s_op),
-                                                 reinterpret_cast<const BlockMap*>(mem + s_op + s_sys + s_st + s_part), d.nblk_apply, s);
-  if (e == hipSuccess) e = launch_fin(FIN_RESID, reinterpret_cast<const SysDesc*>(mem + s_op), d_one, 1, s);
+  hipError_t e = hipMemcpyAsync(mem, &v, sizeof(v), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(p_sys, &d, sizeof(d), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d.st, &st, sizeof(st), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(bmd, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_one, &zero_idx, sizeof(int), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = launch_apply_resid_r64(kAutoTileRows, op_simple(op), reinterpret_cast<const OpView<double>*>(mem),
+                               reinterpret_cast<const SysDesc*>(p_sys), bmd, d.nblk_apply, s);
+  if (e == hipSuccess) e = launch_fin(FIN_RESID, reinterpret_cast<const SysDesc*>(p_sys), d_one, 1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) rc = fail(std::string("mx_residual: ") + hipGetErrorString(e));
-  else if (norm2_out) *norm2_out = st.true_rnorm2;
+  if (e == hipSuccess) e = hipStreamSynchronize(s);   // also keeps the host sources alive until copied
+  if (e != hipSuccess) return fail(std::string(who) + ": " + hipGetErrorString(e));
+  if (norm2_out) *norm2_out = st.true_rnorm2;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream) {
+  SysDesc d;
+  if (resid_setup(op, b, x, r, "mx_residual", &d)) return -1;
+  char* mem = nullptr;
+  MX_TRY(hipMalloc(&mem, resid_layout(d.nblk_apply).total));
+  const int rc = resid_run(*op, d, mem, norm2_out, reinterpret_cast<hipStream_t>(stream), "mx_residual");
   (void)hipFree(mem);
   return rc;
 }
@@ -695,14 +831,18 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
 
   const int chunk = o.chunk;
   const int check = std::min(o.check_every, chunk);
-  if (o.use_graph) {
+  auto capture = [&]() -> int {
+    if (B.gexec) { MX_TRY(hipGraphExecDestroy(B.gexec)); B.gexec = nullptr; }
+    if (B.graph) { MX_TRY(hipGraphDestroy(B.graph)); B.graph = nullptr; }
     MX_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_chunk(B, s, chunk, check);
     hipError_t ce = hipStreamEndCapture(s, &B.graph);
     if (rc) return -1;
     MX_TRY(ce);
     MX_TRY(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
-  }
+    return 0;
+  };
+  if (o.use_graph && capture()) return -1;
   auto launch_chunk = [&]() -> int {
     if (o.use_graph) {
       MX_TRY(hipGraphLaunch(B.gexec, s));
@@ -721,6 +861,29 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
       if (!(B.h_state[slot][i].flags & SF_DONE)) return false;
     return true;
   };
+  // Finished systems leave the launch grids: once the apply blocks of unfinished systems
+  // fall below 90% of those launched, the stream is drained, the maps are rebuilt for
+  // the unfinished systems only and the chunk is captured again.  (A finished system's
+  // blocks would otherwise still be dispatched by every launch, only to exit.)
+  int launched_blocks = B.nb_apply0;
+  auto compact = [&](int slot) -> int {   // returns 1 if rebuilt
+    if (nsys == 1) return 0;
+    std::vector<char> active(nsys, 0);
+    int blocks = 0;
+    for (int i = 0; i < nsys; ++i)
+      if (!(B.h_state[slot][i].flags & SF_DONE)) { active[i] = 1; blocks += B.h_sys[i].nblk_apply; }
+    if (blocks == 0 || blocks > 0.9 * launched_blocks) return 0;
+    MX_TRY(hipStreamSynchronize(s));
+    // a system may have finished in the chunk that was still queued: take the newest states
+    for (int i = 0; i < nsys; ++i) {
+      MX_TRY(hipMemcpy(&B.h_state[slot][i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
+      active[i] = !(B.h_state[slot][i].flags & SF_DONE);
+    }
+    if (layout_groups(B, active)) return -1;
+    launched_blocks = B.nb_apply0;
+    if (B.nb_apply0 > 0 && o.use_graph && capture()) return -1;
+    return 1;
+  };
   // pipelined: keep one chunk queued behind the one whose state is being inspected
   // Safety bound only: every chunk advances each unfinished system by at least one
   // iteration or finishes it (a system that pauses for a replacement idles for the rest
@@ -733,6 +896,14 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     MX_TRY(hipEventSynchronize(B.ev[cur]));
     if (all_done(cur)) break;
     if (c > max_chunks) return fail("mx_solve: iteration limit exceeded without completion (internal error)");
+    const int rebuilt = compact(cur);
+    if (rebuilt < 0) return -1;
+    if (rebuilt) {   // the pipeline restarts from the drained stream
+      if (B.nb_apply0 == 0) break;
+      if (poll(0) || launch_chunk() || poll(1)) return -1;
+      cur = 0;
+      continue;
+    }
     if (launch_chunk() || poll(cur)) return -1;
     cur ^= 1;
   }
