@@ -143,6 +143,15 @@ void mx_plan_destroy(mx_plan* plan);
  * The residual the reference's logging callback prints (fdfd/solvers.py:46-48). */
 int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream);
 
+/* mx_residual without any device allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and bits as mx_residual; its workspace is a fixed
+ * device array of 192 KiB per GPU (enough for every grid mx_residual accepts; a larger
+ * need is refused with an error), held until the process ends.  Every call on one GPU
+ * uses that GPU's single array, so calls that overlap on one GPU (other host threads or
+ * streams) race; the call returns after its stream has finished with the array. */
+int mx_residual_noalloc(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out,
+                        void* stream);
 
 /* Diagonal maps of generic() (fdfd/solvers.py:115,120-139), complex128 in/out:
  *   MX_MAP_RHS      out = p * (c * in)              b = Pl (-i w J): c = -i w

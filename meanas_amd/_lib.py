@@ -119,6 +119,8 @@ PROTOTYPES = {
     'mx_plan_destroy': (None, [c_void_p]),
     'mx_residual': (ctypes.c_int, [ctypes.POINTER(MxOperator), c_void_p, c_void_p, c_void_p,
                                    ctypes.POINTER(ctypes.c_double), c_void_p]),
+    'mx_residual_noalloc': (ctypes.c_int, [ctypes.POINTER(MxOperator), c_void_p, c_void_p, c_void_p,
+                                           ctypes.POINTER(ctypes.c_double), c_void_p]),
     'mx_map': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                               c_void_p, c_void_p, c_void_p]),
     'mx_solve': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.c_int32, ctypes.POINTER(MxSystem),

@@ -10,6 +10,7 @@
 #include <new>
 #include <string>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -701,6 +702,35 @@ int resid_run(const mx_operator& op, SysDesc d, char* mem, double* norm2_out, hi
 
 }  // namespace
 
+// Fixed device workspace of mx_residual_noalloc, one copy per GPU (each device that loads
+// the library holds it until the process ends).  Host code never uses this name as a
+// device address: it takes the address hipGetSymbolAddress reports for the current device.
+constexpr size_t kResidScratchBytes = 192u << 10;
+static __device__ __align__(256) char scratch[kResidScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch`, looked up once per device (with that device current).
+int scratch_for_current_device(char** out) {
+  int dev = -1;
+  MX_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("mx_residual_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("mx_residual_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = reinterpret_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out, void* stream) {
@@ -711,6 +741,19 @@ int mx_residual(const mx_operator* op, const void* b, const void* x, void* r, do
   const int rc = resid_run(*op, d, mem, norm2_out, reinterpret_cast<hipStream_t>(stream), "mx_residual");
   (void)hipFree(mem);
   return rc;
+}
+
+int mx_residual_noalloc(const mx_operator* op, const void* b, const void* x, void* r, double* norm2_out,
+                        void* stream) {
+  SysDesc d;
+  if (resid_setup(op, b, x, r, "mx_residual_noalloc", &d)) return -1;
+  const size_t need = resid_layout(d.nblk_apply).total;
+  if (need > kResidScratchBytes)
+    return fail("mx_residual_noalloc: the grid needs " + std::to_string(need) + " bytes of workspace, more than the " +
+                std::to_string(kResidScratchBytes) + "-byte fixed workspace");
+  char* mem = nullptr;
+  if (scratch_for_current_device(&mem)) return -1;
+  return resid_run(*op, d, mem, norm2_out, reinterpret_cast<hipStream_t>(stream), "mx_residual_noalloc");
 }
 
 int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_options* opt, int32_t reps,

@@ -68,6 +68,8 @@ def test_oversized_cross_section_is_refused_on_the_host(shape):
     norm = ctypes.c_double(0.0)
     assert L.mx_residual(ctypes.byref(op), fake, fake, fake, ctypes.byref(norm), None) != 0
     assert b'apply tiles' in L.mx_last_error(), L.mx_last_error()
+    assert L.mx_residual_noalloc(ctypes.byref(op), fake, fake, fake, ctypes.byref(norm), None) != 0
+    assert b'apply tiles' in L.mx_last_error(), L.mx_last_error()
     sysd = _lib.MxSystem(0, 10, 0, 0, 1e-8, 0.0, fake, fake, fake)
     opts = _lib.MxOptions(_lib.MX_FP64, 0, 0, 0, 0, 1, 0.0, 0, 0, 0, 0)
     stats = (_lib.MxStats * 1)()
