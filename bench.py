@@ -70,6 +70,8 @@ def parse():
     ap.add_argument('--qps', type=float, default=4.0, help='serve: arrival rate')
     ap.add_argument('--form', choices=['auto', 'fused', 'classic'], default='auto',
                     help='COCG iteration form (auto: fused for fp64, classic for mixed)')
+    ap.add_argument('--no-configs', action='store_true',
+                    help='skip the other benchmark workloads (configs 3-5) and the precision study')
     ap.add_argument('--mode', choices=['mixed', 'fp64'], default='fp64',
                     help='solver of the timed step (the other one is the same-GPU comparison)')
     return ap.parse_args()
@@ -280,8 +282,8 @@ def batch_workloads(args):
             b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(w['J']).to(dev), coef=-1j * w['omega'])
             systems.append(dict(op=k, b=b, x=torch.zeros_like(b), rtol=args.rtol, maxiter=200000))
         for op in ops:                                       # autotune outside the timed region
-            from meanas_amd.fdfd.autotune import tuned_slab
-            tuned_slab(op, mode)
+            from meanas_amd.fdfd.autotune import tuned
+            tuned(op, mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         st = solve_batch(ops, systems, mode=mode, options=fopt)
@@ -303,9 +305,9 @@ def batch_workloads(args):
                                 rtol=args.rtol, arrival=r['arrival'], tag=dict(k=k, adjoint=r['adjoint'], shape=r['shape'])))
         srv = Server(capacity=16, mode=mode, chunk=16, options=fopt)
         # warm the autotuner cache for every shape outside the timed run
-        from meanas_amd.fdfd.autotune import tuned_slab
+        from meanas_amd.fdfd.autotune import tuned
         for r in reqs:
-            tuned_slab(srv.prepare(r).op, mode)
+            tuned(DeviceOperator(r.omega, r.dxes, r.epsilon, device=dev), mode)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         res = srv.run(reqs, realtime=True)
@@ -319,6 +321,166 @@ def batch_workloads(args):
                             'mode': args.mode, 'form': args.form, 'capacity': 16})
     line.update(n_gpus=1, dtype='c128' if args.mode == 'fp64' else 'c64 inner + f64 residual', data='synthetic')
     print(json.dumps(line), flush=True)
+
+
+# ---- the other benchmark workloads at full size (extra keys of the default line) -------
+def config3_multi_rhs(args, dev):
+    """16 right-hand sides on the 128x96x96 anisotropic structure in one batch, against
+    the same structure's right-hand sides solved one at a time (first two)."""
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    w = workloads.aniso_multi()
+    op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
+    bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).to(dev), coef=-1j * w['omega']) for J in w['Js']]
+    xs = [torch.zeros_like(b) for b in bs]
+    systems = [dict(op=0, b=b, x=x, rtol=args.rtol, maxiter=200000) for b, x in zip(bs, xs)]
+    solve_batch([op], [dict(systems[0], x=torch.zeros_like(bs[0]))], mode=_lib.MX_FP64)   # warm-up, autotune
+    singles = []
+    for k in range(2):
+        xk = torch.zeros_like(bs[k])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = solve_batch([op], [dict(systems[k], x=xk)], mode=_lib.MX_FP64)[0]
+        torch.cuda.synchronize()
+        singles.append((time.perf_counter() - t0, st['iterations']))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solve_batch([op], systems, mode=_lib.MX_FP64)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    its = [s_['iterations'] for s_ in st]
+    cell_its = op.ncell * sum(its)
+    single_rate = op.ncell * sum(i for _, i in singles) / sum(t for t, _ in singles)
+    out = dict(workload='aniso_multi: 128x96x96, anisotropic box eps, 16 Gaussian sheets, FP64, rtol %g' % args.rtol,
+               solves_per_s=len(bs) / wall, wall_s=wall, converged=sum(s_['status'] == 'converged' for s_ in st),
+               max_rel_residual=max(s_['rel_residual'] for s_ in st), iterations=its,
+               cell_iterations_per_s=cell_its / wall, single_solve_cell_iterations_per_s=single_rate,
+               batch_over_single_rate=cell_its / wall / single_rate,
+               single_solves=[dict(s=round(t, 3), iterations=i) for t, i in singles])
+    log(f'config 3: {out}')
+    return out
+
+
+def config4_uneven(args, dev, form):
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.autotune import tuned
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    probs = workloads.uneven_batch(count=32)
+    ops, systems = [], []
+    for k, w in enumerate(probs):
+        op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], pec=w['pec'], device=dev)
+        ops.append(op)
+        b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(w['J']).to(dev), coef=-1j * w['omega'])
+        systems.append(dict(op=k, b=b, x=torch.zeros_like(b), rtol=args.rtol, maxiter=400000))
+        tuned(op, _lib.MX_FP64)                       # autotune outside the timed region
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = solve_batch(ops, systems, mode=_lib.MX_FP64, options=dict(form=form))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    its = [s_['iterations'] for s_ in st]
+    k = int(numpy.argmax(its))
+    cell_its = sum(op.ncell * i for op, i in zip(ops, its))
+    out = dict(form=form, solves_per_s=len(ops) / wall, wall_s=wall,
+               converged=sum(s_['status'] == 'converged' for s_ in st),
+               max_rel_residual=max(s_['rel_residual'] for s_ in st), cell_iterations_per_s=cell_its / wall,
+               straggler=dict(index=k, shape=ops[k].shape, iterations=its[k]),
+               median_iterations=float(numpy.median(its)), iterations=its)
+    log(f'config 4 ({form}): {out}')
+    del ops, systems
+    return out
+
+
+def config5_serving(args, dev, qps, count):
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.autotune import tuned
+    from meanas_amd.fdfd.device import DeviceOperator
+    from meanas_amd.serve import Request, Server, summarize
+    reqs = []
+    for k, r in enumerate(workloads.serving_stream(count, qps)):
+        reqs.append(Request(omega=r['omega'], dxes=r['dxes'], J=r['J'], epsilon=r['epsilon'], adjoint=r['adjoint'],
+                            rtol=args.rtol, arrival=r['arrival'], tag=dict(k=k, adjoint=r['adjoint'], shape=r['shape'])))
+        tuned(DeviceOperator(r['omega'], r['dxes'], r['epsilon'], device=dev), _lib.MX_FP64)
+    srv = Server(capacity=16, mode=_lib.MX_FP64, chunk=16)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    srv.close()
+    summ = summarize(res, wall)
+    out = dict(qps=qps, requests=count, arrivals_s=reqs[-1].arrival, latency_ms={'p50': summ['p50_ms'], 'p99': summ['p99_ms']},
+               solves_per_s=summ['solves_per_s'], wall_s=wall, converged=summ['converged'],
+               max_rel_residual=max(r.stats['rel_residual'] for r in res), adjoint=summ['adjoint'])
+    log(f'config 5 ({qps} QPS): {out}')
+    return out
+
+
+def precision_study(op, b, args):
+    """Why complex64 Krylov vectors do not pay on this operator: COCG (torch vector ops
+    around the library's complex64 / complex128 apply plans) with complex64 rounding in
+    different places, true FP64 residuals reported.  Also the per-iteration kernel times of
+    the complex64 and complex128 iterations (the speed-up mixed precision could give at
+    equal iteration counts)."""
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import time_kernels
+    p64, p32 = op.plan('c128'), op.plan('c64')
+    c64, c128 = torch.complex64, torch.complex128
+    bn = float(torch.linalg.vector_norm(b))
+
+    def dotu(u, v):
+        return (u.to(c128) * v.to(c128)).sum()
+
+    def true_rel(x):
+        return float(torch.linalg.vector_norm(b - p64.apply(x.to(c128)))) / bn
+
+    def cocg(maxit, store_p=c128, store_x=c128, vec=c128, replace_every=0, report=1000, tol=None):
+        """COCG with r in `vec`; p and x_lo stored in the given precisions; FP64 residual
+        replacement (x += x_lo, r = b - A x, direction kept) every `replace_every` its."""
+        x = torch.zeros_like(b)
+        xlo = torch.zeros_like(b, dtype=store_x)
+        r = b.to(vec).clone()
+        p = r.to(store_p)
+        rho = dotu(r, r)
+        hist, it = [], 0
+        while it < maxit:
+            pd = p.to(c64) if vec == c64 else p.to(c128)
+            q = (p32 if vec == c64 else p64).apply(pd.contiguous())
+            alpha = rho / dotu(pd, q)
+            xlo = (xlo.to(c128) + alpha * pd.to(c128)).to(store_x)
+            r = r - alpha.to(vec) * q
+            rho_new = dotu(r, r)
+            it += 1
+            if replace_every and it % replace_every == 0:
+                x = x + xlo.to(c128)
+                xlo.zero_()
+                r = (b - p64.apply(x)).to(vec)
+                rho_new = dotu(r, r)
+            beta = rho_new / rho
+            rho = rho_new
+            p = (r.to(c128) + beta * pd.to(c128)).to(store_p)
+            if it % report == 0 or it == maxit:
+                rel = true_rel(x + xlo.to(c128))
+                hist.append((it, float('%.3g' % rel)))
+                if not numpy.isfinite(rel) or (tol and rel <= tol):
+                    break
+        return hist
+    t0 = time.perf_counter()
+    out = {
+        'c64_cocg': cocg(6000, store_p=c64, store_x=c64, vec=c64),
+        'fp64_cocg_p_stored_c64': cocg(4000, store_p=c64),
+        'fp64_cocg_x_lo_c64_replaced_every_500': cocg(9000, store_x=c64, replace_every=500, tol=args.rtol),
+        'fp64_cocg': cocg(9000, tol=args.rtol),
+    }
+    kt = {m: time_kernels(op, b, mode, reps=50) for m, mode in (('c64', _lib.MX_MIXED), ('c128', _lib.MX_FP64))}
+    per_it = {m: sum(v for k, v in kt[m].items() if k != 'slab') for m in kt}
+    out['iteration_kernels_us'] = {m: {k: round(v, 2) for k, v in kt[m].items()} for m in kt}
+    out['iteration_us'] = {m: round(v, 2) for m, v in per_it.items()}
+    out['ceiling_at_equal_iterations'] = per_it['c128'] / per_it['c64']
+    out['s'] = time.perf_counter() - t0
+    log(f'precision study: {out}')
+    return out
 
 
 def main():
@@ -467,13 +629,14 @@ def main():
                 'algorithmic_bytes': alg[k], 'timing': 'HIP events on the launch stream (mx_time_kernels)'}
     extra['roofline_apply'] = roof('apply')
     extra['iteration_kernels'] = kern
-    extra['tiling'] = dict(tile_rows=4, slab=kt['slab'])
+    extra['tiling'] = dict(tile_rows=kt['tile_rows'], slab=kt['slab'], chosen_by='autotuner (tile height x slab, in-solve apply time)')
     # complex64 plain apply (the mixed solver's inner operator) for reference
     g = torch.Generator(device=dev).manual_seed(0)
     xa = torch.randn(op.n3, dtype=torch.complex64, device=dev, generator=g)
     ya = torch.empty_like(xa)
-    from meanas_amd.fdfd.autotune import tuned_slab
-    us64 = time_apply(op.plan('c64', slab=tuned_slab(op, _lib.MX_MIXED)), xa, ya, args.apply_reps, stream)
+    from meanas_amd.fdfd.autotune import tuned
+    ty64, slab64 = tuned(op, _lib.MX_MIXED)
+    us64 = time_apply(op.plan('c64', tile_rows=ty64, slab=slab64), xa, ya, args.apply_reps, stream)
     bpc = op.apply_bytes_per_cell(8)
     extra['apply_c64'] = dict(us=us64, bytes_per_cell=bpc, gbs=bpc * op.ncell / (us64 * 1e-6) / 1e9)
 
@@ -525,6 +688,18 @@ def main():
                               if 'value' in cb else ', no projection (iteration count unknown)')
                            + f'; scipy CSR mat-vec is single-threaded, host has {cores} cores'),
                    assembly_s=cb['assembly_s'], s_per_iteration=cb['qmr_s_per_iter'])
+
+    # ---- the other benchmark workloads at full size, and the precision study
+    if not args.no_configs:
+        extra['precision_study'] = precision_study(op, b, args)
+        torch.cuda.empty_cache()
+        cfg = {'config3_multi_rhs': config3_multi_rhs(args, dev)}
+        torch.cuda.empty_cache()
+        cfg['config4_uneven'] = {f: config4_uneven(args, dev, f) for f in ('fused', 'classic')}
+        torch.cuda.empty_cache()
+        cfg['config5_serving'] = {'below_saturation': config5_serving(args, dev, 1.0, 16),
+                                  'saturated': config5_serving(args, dev, 4.0, 40)}
+        extra['configs'] = cfg
 
     ms = elapsed / args.steps * 1e3
     line = {

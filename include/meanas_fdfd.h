@@ -77,7 +77,7 @@ typedef struct mx_system {
   int32_t op;        /* index into the operator array of the call */
   int32_t maxiter;   /* inner iteration limit (reference default: 10 * 3N, scipy qmr) */
   int32_t slab;      /* x-planes per apply block for this system (0 = mx_options.slab / automatic) */
-  int32_t pad;
+  int32_t tile_rows; /* apply tile height for this system: 2, 4 or 8 (0 = mx_options.tile_rows) */
   double rtol, atol; /* stop when ||b - A x|| <= max(rtol ||b||, atol) (scaled system) */
   const void* b;     /* complex128 3N, scaled right-hand side */
   void* x;           /* complex128 3N: in = initial guess (scaled), out = solution (scaled) */

@@ -56,7 +56,7 @@ class MxOperator(ctypes.Structure):
 class MxSystem(ctypes.Structure):
     _fields_ = [
         ('op', ctypes.c_int32), ('maxiter', ctypes.c_int32),
-        ('slab', ctypes.c_int32), ('pad', ctypes.c_int32),
+        ('slab', ctypes.c_int32), ('tile_rows', ctypes.c_int32),
         ('rtol', ctypes.c_double), ('atol', ctypes.c_double),
         ('b', c_void_p), ('x', c_void_p), ('work', c_void_p),
     ]

@@ -209,7 +209,7 @@ std::vector<BlockMap> xcd_order(const std::vector<BlockMap>& natural) {
 // in a single solve (x_lo and r are streamed past the cache by the update).  A group of
 // many small systems also fills the GPU that one of them would leave idle.  A system's
 // block partition does not depend on its group, so results do not either.
-struct Group { int a_off, na, v_off, nv, s_off, ns; };
+struct Group { int a_off, na, v_off, nv, s_off, ns, ty; bool simple; };
 
 // MALL bytes a launch group may fill (MEANAS_AMD_CACHE_MB overrides; 0: one group)
 size_t cache_budget() {
@@ -252,6 +252,7 @@ struct Batch {
   std::vector<Group> groups;
   std::vector<SysDesc> h_sys;
   std::vector<int> h_op;        // operator index of each system
+  std::vector<int> h_ty;        // apply tile height of each system
   const mx_operator* ops = nullptr;
   SysState* h_state[2] = {nullptr, nullptr};   // pinned, double-buffered polls
   hipStream_t stream = nullptr;                  // library-owned capture stream
@@ -264,6 +265,7 @@ struct Batch {
   bool simple = false;          // every operator has simple materials (no mask, no mu, real isotropic eps)
 
   void use_group(const Group& g) {
+    ty = g.ty; simple = g.simple;
     d_bmap_apply = bm_apply0 + g.a_off; nb_apply = g.na;
     d_bmap_vec = bm_vec0 + g.v_off; nb_vec = g.nv;
     d_syslist = syslist0 + g.s_off; nsl = g.ns;
@@ -284,43 +286,69 @@ struct Batch {
   }
 };
 
-// Lay out the block maps and system lists of the `active` systems as consecutive launch
-// groups (see Group) and copy them to the device (synchronous; the caller has drained
-// the stream).  Each group's maps are XCD-ordered on their own.
-int layout_groups(Batch& B, const std::vector<char>& active) {
+// One system as a member of a launch group.
+struct Member {
+  int sys;                  // system (batch) or slot (session) index
+  const mx_operator* op;
+  int op_id;                // systems with the same op_id share materials
+  int ty;                   // apply tile height
+  bool simple;              // simple materials (op_simple)
+  int na, nv;               // apply and vector blocks
+};
+
+// Lay `members` out as consecutive launch groups (see Group): one tile height and one
+// material kind (simple or general) per group, and within those, consecutive members
+// while their hot bytes fit the cache budget.  Each group's maps are XCD-ordered on their
+// own; apply, vec and sl receive the concatenated maps and system lists.
+void make_groups(const std::vector<Member>& members, int mode, std::vector<BlockMap>& apply,
+                 std::vector<BlockMap>& vec, std::vector<int>& sl, std::vector<Group>& groups) {
   const size_t budget = cache_budget();
+  apply.clear(); vec.clear(); sl.clear(); groups.clear();
+  for (int ty : {2, 4, 8})
+    for (int simple = 1; simple >= 0; --simple) {
+      std::vector<BlockMap> ga, gv;
+      std::vector<int> gs, gops;
+      size_t bytes = 0;
+      auto close = [&]() {
+        if (gs.empty()) return;
+        Group g;
+        g.a_off = (int)apply.size(); g.v_off = (int)vec.size(); g.s_off = (int)sl.size();
+        const std::vector<BlockMap> oa = xcd_order(ga), ov = xcd_order(gv);
+        apply.insert(apply.end(), oa.begin(), oa.end());
+        vec.insert(vec.end(), ov.begin(), ov.end());
+        sl.insert(sl.end(), gs.begin(), gs.end());
+        g.na = (int)oa.size(); g.nv = (int)ov.size(); g.ns = (int)gs.size();
+        g.ty = ty; g.simple = simple != 0;
+        groups.push_back(g);
+        ga.clear(); gv.clear(); gs.clear(); gops.clear();
+        bytes = 0;
+      };
+      for (const Member& m : members) {
+        if (m.ty != ty || (int)m.simple != simple) continue;
+        const bool new_op = std::find(gops.begin(), gops.end(), m.op_id) == gops.end();
+        const size_t add = vec_hot_bytes(*m.op, mode) + (new_op ? mat_hot_bytes(*m.op, mode) : 0);
+        if (budget > 0 && !gs.empty() && bytes + add > budget) close();
+        if (std::find(gops.begin(), gops.end(), m.op_id) == gops.end()) gops.push_back(m.op_id);
+        bytes += add;
+        gs.push_back(m.sys);
+        for (int b = 0; b < m.na; ++b) ga.push_back({m.sys, b});
+        for (int b = 0; b < m.nv; ++b) gv.push_back({m.sys, b});
+      }
+      close();
+    }
+}
+
+// Lay out the launch groups of the `active` systems of a batch and copy the maps to the
+// device (synchronous; the caller has drained the stream).
+int layout_groups(Batch& B, const std::vector<char>& active) {
+  std::vector<Member> members;
+  for (int s = 0; s < B.nsys; ++s)
+    if (active[s])
+      members.push_back({s, &B.ops[B.h_op[s]], B.h_op[s], B.h_ty[s], op_simple(B.ops[B.h_op[s]]),
+                         B.h_sys[s].nblk_apply, B.h_sys[s].nblk_vec});
   std::vector<BlockMap> apply, vec;
   std::vector<int> sl;
-  B.groups.clear();
-  std::vector<BlockMap> ga, gv;
-  std::vector<int> gs, gops;
-  size_t bytes = 0;
-  auto close = [&]() {
-    if (gs.empty()) return;
-    Group g;
-    g.a_off = (int)apply.size(); g.v_off = (int)vec.size(); g.s_off = (int)sl.size();
-    const std::vector<BlockMap> oa = xcd_order(ga), ov = xcd_order(gv);
-    apply.insert(apply.end(), oa.begin(), oa.end());
-    vec.insert(vec.end(), ov.begin(), ov.end());
-    sl.insert(sl.end(), gs.begin(), gs.end());
-    g.na = (int)oa.size(); g.nv = (int)ov.size(); g.ns = (int)gs.size();
-    B.groups.push_back(g);
-    ga.clear(); gv.clear(); gs.clear(); gops.clear();
-    bytes = 0;
-  };
-  for (int s = 0; s < B.nsys; ++s) {
-    if (!active[s]) continue;
-    const mx_operator& op = B.ops[B.h_op[s]];
-    const bool new_op = std::find(gops.begin(), gops.end(), B.h_op[s]) == gops.end();
-    const size_t add = vec_hot_bytes(op, B.mode) + (new_op ? mat_hot_bytes(op, B.mode) : 0);
-    if (budget > 0 && !gs.empty() && bytes + add > budget) close();
-    if (std::find(gops.begin(), gops.end(), B.h_op[s]) == gops.end()) gops.push_back(B.h_op[s]);
-    bytes += add;
-    gs.push_back(s);
-    for (int b = 0; b < B.h_sys[s].nblk_apply; ++b) ga.push_back({s, b});
-    for (int b = 0; b < B.h_sys[s].nblk_vec; ++b) gv.push_back({s, b});
-  }
-  close();
+  make_groups(members, B.mode, apply, vec, sl, B.groups);
   B.nb_apply0 = (int)apply.size();
   B.nb_vec0 = (int)vec.size();
   B.nsl0 = (int)sl.size();
@@ -337,18 +365,21 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
   B.nops = nops;
   B.nsys = nsys;
   B.ops = ops;
-  const int ty = o.tile_rows ? o.tile_rows : kAutoTileRows;
-  if (ty != 2 && ty != 4 && ty != 8) return fail("tile_rows must be 2, 4 or 8");
-  B.ty = ty;
+  const int ty_default = o.tile_rows ? o.tile_rows : kAutoTileRows;
+  if (ty_default != 2 && ty_default != 4 && ty_default != 8) return fail("tile_rows must be 2, 4 or 8");
+  B.ty = ty_default;
   for (int i = 0; i < nops; ++i)
     if (!valid_op(ops[i], o.mode, &why)) return fail("operator " + std::to_string(i) + ": " + why);
 
   size_t total_apply = 0, total_vec = 0;
   B.h_sys.resize(nsys);
   B.h_op.resize(nsys);
+  B.h_ty.resize(nsys);
   for (int s = 0; s < nsys; ++s) {
     const mx_system& y = sys[s];
     if (y.op < 0 || y.op >= nops) return fail("system " + std::to_string(s) + ": bad operator index");
+    const int ty = y.tile_rows ? y.tile_rows : ty_default;
+    if (ty != 2 && ty != 4 && ty != 8) return fail("system " + std::to_string(s) + ": tile_rows must be 2, 4 or 8");
     if (!y.b || !y.x || !y.work) return fail("system " + std::to_string(s) + ": b, x and work are required");
     const mx_operator& op = ops[y.op];
     const WorkLayout w = work_layout(op, o.mode);
@@ -376,6 +407,7 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
     B.h_sys[s] = d;
     B.h_op[s] = y.op;
+    B.h_ty[s] = ty;
     total_apply += d.nblk_apply;
     total_vec += d.nblk_vec;
   }
@@ -466,6 +498,19 @@ int enqueue_group_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
     }
   }
   return 0;
+}
+
+// residual replacement on the paused systems of every launch group
+int enqueue_replacement_groups(Batch& B, hipStream_t s) {
+  if (B.groups.empty()) return enqueue_replacement(B, s);
+  int rc = 0;
+  for (const Group& g : B.groups) {
+    B.use_group(g);
+    rc = enqueue_replacement(B, s);
+    if (rc) break;
+  }
+  B.use_all();
+  return rc;
 }
 
 // `iters` iterations of every launch group, one group after the other (a batch view
@@ -766,6 +811,7 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
   y.op = 0;
   Batch B;
   if (build_batch(B, op, 1, &y, 1, o)) return -1;
+  B.use_group(B.groups[0]);   // one system: one launch group
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const bool fp64 = o.mode == MX_FP64;
   // set-up as in mx_solve, with every stopping and pausing rule off, so each kernel
@@ -870,7 +916,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   if (launch_zero_slot(B.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) return fail("zero x_lo");
   MX_TRY(launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   MX_TRY(launch_fin(FIN_BNORM, B.d_sys, B.d_syslist, B.nsl, s));
-  if (enqueue_replacement(B, s)) return -1;
+  if (enqueue_replacement_groups(B, s)) return -1;
 
   const int chunk = o.chunk;
   const int check = std::min(o.check_every, chunk);
@@ -1015,6 +1061,8 @@ struct mx_session {
   std::vector<mx_operator> h_ops;
   std::vector<char> active;
   std::vector<char> restored;            // best iterate already restored for a finished slot
+  std::vector<int> h_ty;                 // apply tile height of each slot
+  std::vector<Group> groups;             // launch groups of the active slots (offsets into d_bm)
   int ty = 4;
   int nb_apply = 0, nb_vec = 0;
   bool dirty = true;
@@ -1064,47 +1112,48 @@ int ensure_staging(mx_session* S, size_t need) {
   return 0;
 }
 
-// Batch view of a session (shares its device tables) for the enqueue helpers.
+// Batch view of a session (shares its device tables) for the enqueue helpers: the
+// maps of the admitted slot (one group of tile height ty), or with `groups` the launch
+// groups of every active slot.
 Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const BlockMap* bm_vec, int nb_vec,
-                    const int* syslist, int nsl) {
+                    const int* syslist, int nsl, int ty, bool simple, const std::vector<Group>* groups = nullptr) {
   Batch B;
-  B.d_syslist = const_cast<int*>(syslist);
-  B.nsl = nsl;
   B.mode = S->o.mode;
-  B.ty = S->ty;
   B.d_ops32 = S->d_ops32;
   B.d_ops64 = S->d_ops64;
   B.d_sys = S->d_sys;
-  B.d_bmap_apply = const_cast<BlockMap*>(bm_apply);
-  B.d_bmap_vec = const_cast<BlockMap*>(bm_vec);
-  B.nb_apply = nb_apply;
-  B.nb_vec = nb_vec;
-  B.simple = true;
-  for (int i = 0; i < S->capacity; ++i)
-    if (S->active[i]) B.simple = B.simple && op_simple(S->h_ops[i]);
+  B.bm_apply0 = const_cast<BlockMap*>(bm_apply);
+  B.bm_vec0 = const_cast<BlockMap*>(bm_vec);
+  B.syslist0 = const_cast<int*>(syslist);
+  B.nb_apply0 = nb_apply;
+  B.nb_vec0 = nb_vec;
+  B.nsl0 = nsl;
+  if (groups) B.groups = *groups;
+  B.use_all();
+  B.ty = ty;
+  B.simple = simple;
   return B;
 }
 
 void release_view(Batch& B) {   // the view owns nothing
   B.d_mem = nullptr;
   B.d_syslist = nullptr;
+  B.syslist0 = nullptr;
   B.stream = nullptr;
   B.ev[0] = B.ev[1] = nullptr;
   B.h_state[0] = B.h_state[1] = nullptr;
 }
 
 int session_rebuild(mx_session* S) {
+  std::vector<Member> members;
+  for (int i = 0; i < S->capacity; ++i)
+    if (S->active[i])
+      members.push_back({i, &S->h_ops[i], i, S->h_ty[i], op_simple(S->h_ops[i]), S->h_sys[i].nblk_apply,
+                         S->h_sys[i].nblk_vec});
   std::vector<BlockMap> a, v;
   std::vector<int> sl;
-  for (int i = 0; i < S->capacity; ++i) {
-    if (!S->active[i]) continue;
-    sl.push_back(i);
-    for (int b = 0; b < S->h_sys[i].nblk_apply; ++b) a.push_back({i, b});
-    for (int b = 0; b < S->h_sys[i].nblk_vec; ++b) v.push_back({i, b});
-  }
+  make_groups(members, S->o.mode, a, v, sl, S->groups);
   S->nsl = (int)sl.size();
-  a = xcd_order(a);
-  v = xcd_order(v);
   S->nb_apply = (int)a.size();
   S->nb_vec = (int)v.size();
   const size_t need = a.size() + v.size();
@@ -1138,6 +1187,7 @@ mx_session* mx_session_create(int32_t capacity, const mx_options* opt) {
   S->h_ops.assign(capacity, mx_operator{});
   S->active.assign(capacity, 0);
   S->restored.assign(capacity, 0);
+  S->h_ty.assign(capacity, S->ty);
   const size_t s32 = align_up(sizeof(OpView<float>) * capacity);
   const size_t s64 = align_up(sizeof(OpView<double>) * capacity);
   const size_t ss = align_up(sizeof(SysDesc) * capacity);
@@ -1173,10 +1223,12 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   SysDesc d;
   std::memset(&d, 0, sizeof(d));
   d.op = slot;   // one operator per slot
+  const int ty = y->tile_rows ? y->tile_rows : S->ty;
+  if (ty != 2 && ty != 4 && ty != 8) return fail("tile_rows must be 2, 4 or 8");
   d.slab = y->slab > 0 ? std::min<int>(y->slab, op->nx)
-         : S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, S->ty, true);
-  if (d.slab < 1) return fail(too_many_tiles(*op, S->ty));
-  d.nblk_apply = apply_blocks(*op, S->ty, d.slab);
+         : S->o.slab > 0 ? std::min<int>(S->o.slab, op->nx) : auto_slab(*op, ty, true);
+  if (d.slab < 1) return fail(too_many_tiles(*op, ty));
+  d.nblk_apply = apply_blocks(*op, ty, d.slab);
   d.n3 = 3 * ncell_of(*op);
   d.nblk_vec = vec_blocks(d.n3);
   if (d.nblk_apply > kMaxPartials) return fail("too many apply blocks for one system; use a deeper slab");
@@ -1192,6 +1244,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
   S->h_sys[slot] = d;
   S->h_ops[slot] = *op;
+  S->h_ty[slot] = ty;
 
   // tables and initial state, staged through pinned memory owned by the session
   MX_TRY(hipStreamSynchronize(s));
@@ -1228,7 +1281,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   MX_TRY(hipMemcpy(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
   int* adm_sl = S->d_syslist + S->capacity;
   MX_TRY(hipMemcpy(adm_sl, &slot, sizeof(int), hipMemcpyHostToDevice));
-  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1);
+  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1, ty, op_simple(*op));
   int rc = 0;
   if (launch_zero_slot(S->o.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) rc = fail("zero x_lo");
   if (!rc && launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s) != hipSuccess) rc = fail("bnorm");
@@ -1247,7 +1300,8 @@ int mx_session_step(mx_session* S, int32_t iterations, void* stream) {
   (void)stream;
   if (S->dirty && session_rebuild(S)) return -1;
   if (S->nb_apply == 0 || iterations <= 0) return 0;
-  Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec, S->d_syslist, S->nsl);
+  Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec, S->d_syslist, S->nsl, S->ty,
+                          false, &S->groups);
   const int check = std::min<int>(S->o.check_every, iterations);
   int rc = 0;
   if (S->o.use_graph) {

@@ -1,15 +1,21 @@
-"""Autotuner for the apply kernel's plane-block depth (x-planes per block).
+"""Autotuner for the apply kernel's tile shape: tile height (y rows per block) and slab
+depth (x-planes per block).
 
-Each block of the apply kernel owns a 4 x 64 (y, z) tile and marches over `slab`
-x-planes; a deeper slab amortises the two extra planes a block reads at its start,
-a shallower one gives more blocks to fill the 256 CUs.  The best depth depends on the
-grid shape and the precision, so it is measured: every candidate depth is timed with
-an apply plan on the real operator (HIP events on one stream) and the fastest is kept,
-per (device, shape, precision), in memory and in a JSON cache file.
+Each block of the apply kernel owns a TY x 64 (y, z) tile and marches over `slab`
+x-planes.  A deeper slab amortises the two extra planes a block reads at its start; a
+shallower one, or a lower tile, gives more blocks to fill the 256 CUs; the register
+budget of the kernel decides how many of them are resident.  The best shape depends on
+the grid, the precision and the material kinds, so it is measured: every candidate
+(tile height, slab) is timed with `mx_time_kernels`, i.e. the solver's own iteration
+launched in solve order on the real operator (so the caches hold what they hold in a
+solve), and the fastest apply is kept per (device, shape, precision, material flags), in
+memory and in a JSON cache file.
 
-The tile height stays fixed (4 rows): a system's block partition -- and so the order
-of its reductions -- must depend on that system alone, never on the rest of a batch.
-Set MEANAS_AMD_AUTOTUNE=0 to use the library's built-in depth rule instead.
+A system's block partition -- and so the order of its reductions -- depends on that
+system alone (its shape and materials through this table), never on the rest of a
+batch, so a system gives the same bits alone and in any batch.
+Set MEANAS_AMD_AUTOTUNE=0 to use the library's built-in rule (4-row tiles, automatic
+slab) instead.
 """
 from __future__ import annotations
 
@@ -21,19 +27,33 @@ import torch
 
 from .. import _lib
 
-CANDIDATES = (2, 4, 8, 16, 32)
-TILE_ROWS = 4
+TILE_ROWS = (2, 4, 8)
+SLABS = (2, 4, 6, 8, 11, 12, 16, 24, 32)
 MAX_PARTIALS = 4096       # per-system reduction limit of the finishing kernel
-CACHE_PATH = os.path.join(os.path.expanduser('~'), '.cache', 'meanas_amd', 'autotune.json')
+CACHE_PATH = os.path.join(os.path.expanduser('~'), '.cache', 'meanas_amd', 'autotune2.json')
 
 _lock = threading.Lock()
-_memo: dict[str, int] = {}
+_memo: dict[str, tuple[int, int]] = {}
 _loaded = False
 
 
-def _blocks(shape, slab: int) -> int:
+def blocks(shape, tile_rows: int, slab: int) -> int:
     nx, ny, nz = shape
-    return ((ny + TILE_ROWS - 1) // TILE_ROWS) * ((nz + 63) // 64) * ((nx + slab - 1) // slab)
+    return ((ny + tile_rows - 1) // tile_rows) * ((nz + 63) // 64) * ((nx + slab - 1) // slab)
+
+
+def candidates(shape) -> list[tuple[int, int]]:
+    """Admissible (tile height, slab) pairs: slab no deeper than the grid (the shallowest
+    candidate always), and at most MAX_PARTIALS blocks."""
+    out = []
+    for ty in TILE_ROWS:
+        for slab in SLABS:
+            if slab > shape[0] and slab != SLABS[0]:
+                continue
+            if blocks(shape, ty, slab) > MAX_PARTIALS:
+                continue
+            out.append((ty, slab))
+    return out
 
 
 def _key(op, mode: int) -> str:
@@ -48,8 +68,8 @@ def _load():
     _loaded = True
     try:
         with open(CACHE_PATH) as f:
-            _memo.update({k: int(v) for k, v in json.load(f).items()})
-    except (OSError, ValueError):
+            _memo.update({k: (int(v[0]), int(v[1])) for k, v in json.load(f).items()})
+    except (OSError, ValueError, TypeError, IndexError):
         pass
 
 
@@ -58,41 +78,29 @@ def _save():
         os.makedirs(os.path.dirname(CACHE_PATH), exist_ok=True)
         tmp = CACHE_PATH + '.tmp'
         with open(tmp, 'w') as f:
-            json.dump(_memo, f, indent=0, sort_keys=True)
+            json.dump({k: list(v) for k, v in _memo.items()}, f, indent=0, sort_keys=True)
         os.replace(tmp, CACHE_PATH)
     except OSError:
         pass
 
 
-def measure(op, mode: int, reps: int = 10) -> dict[int, float]:
-    """Microseconds per apply for every admissible candidate depth."""
-    dtype = torch.complex64 if mode == _lib.MX_MIXED else torch.complex128
-    x = torch.randn(op.n3, dtype=dtype, device=op.device)
-    y = torch.empty_like(x)
-    stream = torch.cuda.current_stream(op.device)
+def measure(op, mode: int, reps: int = 8) -> dict[tuple[int, int], float]:
+    """Microseconds per in-solve apply for every admissible (tile height, slab)."""
+    from .device import time_kernels
+    g = torch.Generator(device=op.device).manual_seed(0)
+    b = torch.randn(op.n3, dtype=torch.complex128, device=op.device, generator=g)
     out = {}
-    for slab in CANDIDATES:
-        if slab > op.shape[0] and slab != CANDIDATES[0]:
-            continue
-        if _blocks(op.shape, slab) > MAX_PARTIALS:
-            continue
-        plan = op.plan('c64' if mode == _lib.MX_MIXED else 'c128', tile_rows=TILE_ROWS, slab=slab)
-        for _ in range(2):
-            plan.apply(x, y, stream=stream)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            plan.apply(x, y, stream=stream)
-        e1.record(stream)
-        e1.synchronize()
-        out[slab] = e0.elapsed_time(e1) * 1e3 / reps
+    for ty, slab in candidates(op.shape):
+        kt = time_kernels(op, b, mode, reps=reps, slab=slab, tile_rows=ty)
+        out[(ty, slab)] = kt['apply']
     return out
 
 
-def tuned_slab(op, mode: int) -> int:
-    """Best measured slab depth for this operator's shape and precision (0 = library rule)."""
+def tuned(op, mode: int) -> tuple[int, int]:
+    """Best measured (tile height, slab) for this operator's shape, precision and
+    material kinds; (0, 0) means the library's built-in rule."""
     if os.environ.get('MEANAS_AMD_AUTOTUNE', '1') == '0':
-        return 0
+        return 0, 0
     key = _key(op, mode)
     with _lock:
         _load()
@@ -100,8 +108,13 @@ def tuned_slab(op, mode: int) -> int:
             return _memo[key]
         times = measure(op, mode)
         if not times:
-            return 0
+            return 0, 0
         best = min(times, key=times.get)
         _memo[key] = best
         _save()
         return best
+
+
+def tuned_slab(op, mode: int) -> int:
+    """The slab depth of `tuned` (its tile height goes with it: use `tuned` for both)."""
+    return tuned(op, mode)[1]

@@ -351,11 +351,11 @@ def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
         if work is None:
             work = torch.empty(op.work_bytes(mode), dtype=torch.uint8, device=op.device)
         keep.append(work)
-        slab = int(s.get('slab', 0))
-        if slab == 0 and o['slab'] == 0 and o.get('autotune', True):
-            from .autotune import tuned_slab
-            slab = tuned_slab(op, mode)
-        sysarr[i] = _lib.MxSystem(s['op'], int(s.get('maxiter', 10 * op.n3)), slab, 0, float(s.get('rtol', 1e-5)),
+        slab, ty = int(s.get('slab', 0)), int(s.get('tile_rows', 0))
+        if slab == 0 and o['slab'] == 0 and o['tile_rows'] == 0 and ty == 0 and o.get('autotune', True):
+            from .autotune import tuned
+            ty, slab = tuned(op, mode)
+        sysarr[i] = _lib.MxSystem(s['op'], int(s.get('maxiter', 10 * op.n3)), slab, ty, float(s.get('rtol', 1e-5)),
                                   float(s.get('atol', 0.0)), s['b'].data_ptr(), s['x'].data_ptr(), work.data_ptr())
     opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'], o['delta'],
                           o['stall_iters'], o['stall_limit'], _lib.FORMS[o['form']])
@@ -372,16 +372,18 @@ def time_kernels(op: DeviceOperator, b: torch.Tensor, mode: int = _lib.MX_FP64, 
                  slab: int | None = None, stream=None, tile_rows: int = 0, form: str = 'auto') -> dict:
     """Average microseconds per launch of each kernel of one solver iteration
     (`mx_time_kernels`), timed with HIP events on `stream` on the system A x = b.
-    The tiling is the one a solve of this operator uses (autotuned slab unless given)."""
+    The tiling is the one a solve of this operator uses (the autotuned tile height and slab
+    depth unless given)."""
     op._check_vec(b, torch.complex128)
     x = torch.zeros_like(b)
     work = torch.empty(op.work_bytes(mode), dtype=torch.uint8, device=op.device)
     if slab is None:
-        from .autotune import tuned_slab
-        slab = tuned_slab(op, mode)
-    sysd = _lib.MxSystem(0, 1, int(slab), 0, 0.0, 0.0, b.data_ptr(), x.data_ptr(), work.data_ptr())
+        from .autotune import tuned
+        ty, slab = tuned(op, mode)
+        tile_rows = tile_rows or ty
+    sysd = _lib.MxSystem(0, 1, int(slab), int(tile_rows), 0.0, 0.0, b.data_ptr(), x.data_ptr(), work.data_ptr())
     opts = _lib.MxOptions(mode, int(tile_rows), 0, 0, 0, 1, 0.0, 0, 0, _lib.FORMS[form])
     us = (ctypes.c_double * len(KERNEL_NAMES))()
     _lib.check(_lib.lib().mx_time_kernels(ctypes.byref(op.desc), ctypes.byref(sysd), ctypes.byref(opts), int(reps),
                                           us, len(KERNEL_NAMES), torch_stream_handle(stream)), 'mx_time_kernels')
-    return dict(zip(KERNEL_NAMES, list(us)), slab=int(slab))
+    return dict(zip(KERNEL_NAMES, list(us)), slab=int(slab), tile_rows=int(tile_rows))

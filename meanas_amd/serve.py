@@ -174,8 +174,9 @@ class Server:
         maxiter = req.maxiter if req.maxiter is not None else 10 * slot.op.n3
         if slot.prior:
             maxiter = max(1, maxiter - slot.prior['iterations'])
-        from .fdfd.autotune import tuned_slab
-        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), tuned_slab(slot.op, slot.mode), 0, req.rtol, req.atol,
+        from .fdfd.autotune import tuned
+        ty, slab = tuned(slot.op, slot.mode)
+        sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), slab, ty, req.rtol, req.atol,
                              slot.b.data_ptr(), slot.x.data_ptr(), slot.work.data_ptr())
         h = self._session(slot.mode)
         idx = _lib.lib().mx_session_admit(h, ctypes.byref(slot.op.desc), ctypes.byref(sysd), torch_stream_handle())

@@ -186,3 +186,14 @@ def test_boundary_mask_matches_oracle(periodic):
     want = ref.e_boundary_source(m, 0.3, dxes, eps, periodic_mask_edges=periodic)
     got = sparse.diags_array(keep) @ full
     assert abs(got - want).max() == 0
+
+
+def test_autotuner_candidates_respect_the_partial_limit():
+    """The tuner's (tile height, slab) candidates stay within the per-system partial-sum
+    limit and the grid depth; a cross-section beyond the limit has none."""
+    from meanas_amd.fdfd import autotune
+    c = autotune.candidates((128, 128, 128))
+    assert (4, 16) in c and (4, 11) in c and (2, 32) in c
+    assert all(autotune.blocks((128, 128, 128), ty, s) <= autotune.MAX_PARTIALS for ty, s in c)
+    assert all(s <= 8 or s == autotune.SLABS[0] for _, s in autotune.candidates((8, 48, 48)))
+    assert autotune.candidates((1, 2048, 2048)) == []
