@@ -606,11 +606,14 @@ def main():
     n3 = op.n3
     alg = {   # algorithmic HBM bytes per launch
         'apply': op.apply_bytes_per_cell(elem) * op.ncell,              # p read, q written, materials
-        'update': (7 if fp64 else 6) * elem * n3,                       # x, r, p, q read; x, r (, p) written
+        # fused FP64: r, p, q read, r, p written, and x_lo read and written every other
+        # launch (the deferred solution update): 6 streams on average; classic complex64:
+        # x_lo, r, p, q read, x_lo, r written
+        'update': 6 * elem * n3,
         'pupdate': 0 if fp64 else 3 * elem * n3,                        # fused form: no-op launch
     }
     names = {'apply': f'k_apply<{"double" if fp64 else "float"}, COCG epilogue>',
-             'update': f'k_update<{"double" if fp64 else "float"}>{" (fused direction update)" if fp64 else ""}',
+             'update': f'k_update<{"double" if fp64 else "float"}>{" (fused direction update, x_lo every other launch)" if fp64 else ""}',
              'pupdate': f'k_pupdate<{"double" if fp64 else "float"}>'}
     it_us = sum(kt[k] for k in ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate'))
     kern = {k: dict(us=round(kt[k], 2), share=round(kt[k] / it_us, 3)) for k in

@@ -172,7 +172,13 @@ enum : int {
   SF_SWITCH_EXIT = 4096,  // on losing track, finish (SF_SWITCHED) instead of switching form in place
   SF_SWITCHED  = 8192, // finished to be continued in the classic form by the host
   SF_UPD_PENDING = 16384, // the update ran; its partials await the post-update finishing step
+  SF_XPEND     = 32768, // fused form: x_lo still lacks the last alpha p, kept as xa (p - r)
+  SF_XSKIP     = 65536, // fused form: the coming update pass leaves x_lo alone (it defers alpha p)
 };
+
+// Deferred solution update of the fused form: an update pass may leave x_lo alone when
+// |beta|^2 is at least this (alpha p is then recovered one pass later from p' - r' = beta p)
+constexpr double kDeferBeta2 = 1e-6;
 
 // |r^T r| below this fraction of ||r||^2 counts as a COCG (Lanczos) breakdown
 constexpr double kRhoBreakdown = 1e-10;
@@ -207,6 +213,9 @@ struct SysState {
                            // >= 100x below the true one (precision exhausted)
   int stall_limit;         // stop (SF_STALL) after this many such replacements
   int mark_iter;           // iteration of the last 2x improvement
+  int pad_;
+  cplx<double> xa;         // fused form: the deferred alpha p equals xa (p - r) (SF_XPEND)
+  cplx<double> xc1, xc2;   // fused form: the coming update pass adds xc1 p + xc2 r to x_lo
 };
 
 // Finishing-kernel kinds (k_fin<KIND>, vector.hip)

@@ -12,6 +12,10 @@
 //   k_fin<0>  : alpha = rho / mu; beta = (alpha^2 q^T q - rho) / rho
 //   k_update  : x_lo += alpha p, r -= alpha q, p = r + beta p; rho' = r^T r, ||r||^2
 //   k_fin<1>  : rho = rho', convergence and replacement checks
+// x_lo is only needed at a replacement, so every other pass leaves it alone: a pass
+// with beta away from zero defers its alpha p, which the next pass recovers as
+// (alpha / beta) (p' - r') and adds together with its own term (8 vector streams per
+// iteration on average instead of 9; k_accum flushes a pending term).
 // beta is the value that makes the new p A-conjugate to the old one: with r^T A p =
 // p^T A p (conjugacy of the previous direction), r'^T r' = rho - 2 alpha mu + alpha^2
 // q^T q = alpha^2 q^T q - rho.  Knowing beta before r' is reduced lets the direction
@@ -167,7 +171,10 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
 }
 
 // ---- x_lo += alpha p ; r -= alpha q ; [p = r + beta p] ; rho, ||r||^2 ----------------
-template <typename T, bool kFused>
+// kFused: the direction update rides along; x_lo += xc1 p + xc2 r (r before this pass's
+// update; xc1 = alpha, xc2 = 0 unless a deferred term is added), or with kSkipX x_lo is
+// left alone (the fused form's deferred solution update).  Classic: x_lo += alpha p.
+template <typename T, bool kFused, bool kSkipX = false>
 __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm) {
   __shared__ double s_red[3 * 16];
   SysState* st = S.st;
@@ -177,10 +184,15 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm)
   const cplx<T>* __restrict__ q = reinterpret_cast<const cplx<T>*>(S.vec[VS_Q]);
   const cplx<T> alpha = cast<T>(gload<double>(&st->alpha, 0));
   const cplx<T> beta = kFused ? cast<T>(gload<double>(&st->beta, 0)) : zero<T>();
+  const cplx<T> xc1 = kFused ? cast<T>(gload<double>(&st->xc1, 0)) : alpha;
+  const cplx<T> xc2 = kFused ? cast<T>(gload<double>(&st->xc2, 0)) : zero<T>();
   const long long n = S.n3;
   T rr_re = 0, rr_im = 0, nr = 0;
   auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T>& pv, cplx<T> qv) {
-    xv = xv + alpha * pv;
+    if constexpr (!kSkipX) {
+      if constexpr (kFused) xv = xv + xc1 * pv + xc2 * rv;
+      else xv = xv + alpha * pv;
+    }
     rv = rv - alpha * qv;
     if constexpr (kFused) pv = rv + beta * pv;
     rr_re += rv.re * rv.re - rv.im * rv.im;
@@ -195,7 +207,7 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm)
 #pragma unroll
     for (int u = 0; u < VU; ++u)
       if (t.ok(u)) {
-        xv[u] = ld<T, kStreamXR>(x, t.at(u));
+        if constexpr (!kSkipX) xv[u] = ld<T, kStreamXR>(x, t.at(u));
         rv[u] = ld<T, kStreamXR>(r, t.at(u));
         pv[u] = ld<T>(p, t.at(u));
         qv[u] = ld<T>(q, t.at(u));
@@ -206,15 +218,15 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm)
 #pragma unroll
         for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
         // x and r are next read one iteration later; p is read by the next kernel
-        stp<T, kStreamXR>(x, t.at(u), xv[u]);
+        if constexpr (!kSkipX) stp<T, kStreamXR>(x, t.at(u), xv[u]);
         stp<T, kStreamXR>(r, t.at(u), rv[u]);
         if constexpr (kFused) stp<T>(p, t.at(u), pv[u]);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
-    cplx<T> xe = x[n - 1], re = r[n - 1], pe1 = p[n - 1];
+    cplx<T> xe = kSkipX ? zero<T>() : x[n - 1], re = r[n - 1], pe1 = p[n - 1];
     one(xe, re, pe1, q[n - 1]);
-    x[n - 1] = xe;
+    if constexpr (!kSkipX) x[n - 1] = xe;
     r[n - 1] = re;
     if constexpr (kFused) p[n - 1] = pe1;
   }
@@ -229,8 +241,12 @@ __global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ syste
   const SysDesc& S = systems[bm.sys];
   const int f = gload1<int>(&S.st->flags, 0);
   if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
-  if (sizeof(T) == 8 && (f & SF_FUSED)) update_body<T, true>(S, bm);   // the direction update rides along
-  else update_body<T, false>(S, bm);
+  if (sizeof(T) == 8 && (f & SF_FUSED)) {   // the direction update rides along
+    if (f & SF_XSKIP) update_body<T, true, true>(S, bm);
+    else update_body<T, true, false>(S, bm);
+  } else {
+    update_body<T, false>(S, bm);
+  }
 }
 
 // ---- x (c128) += x_lo ; x_lo = 0   (paused systems) --------------------------------
@@ -238,10 +254,27 @@ template <typename T>
 __global__ __launch_bounds__(VB) void k_accum(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
-  if (skip_sys(S.st->flags, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
+  const int f = S.st->flags;
+  if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
   cplx<double>* __restrict__ x = reinterpret_cast<cplx<double>*>(S.vec[VS_X]);
   cplx<T>* __restrict__ xl = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   const long long n = S.n3;
+  if (sizeof(T) == 8 && (f & SF_XPEND)) {
+    // the fused form's deferred alpha p = xa (p - r), recovered before the replacement
+    // overwrites r and p: x += x_lo + xa (p - r), x_lo = 0
+    const cplx<double> xa = gload<double>(&S.st->xa, 0);
+    const cplx<double>* __restrict__ pp = reinterpret_cast<const cplx<double>*>(S.vec[VS_P]);
+    const cplx<double>* __restrict__ rr = reinterpret_cast<const cplx<double>*>(S.vec[VS_R]);
+    cplx<double>* __restrict__ xl64 = reinterpret_cast<cplx<double>*>(S.vec[VS_XLO]);
+    long long pb, pe;
+    block_range(n, bm.local, S.nblk_vec, &pb, &pe);
+    for (long long e = pb + threadIdx.x; e < pe; e += VB) {
+      const cplx<double> d = xa * (gload<double>(pp, e) - gload<double>(rr, e));
+      gstore<double>(x, e, gload<double>(x, e) + (gload<double>(xl64, e) + d));
+      gstore<double>(xl64, e, zero<double>());
+    }
+    return;
+  }
   constexpr int K = Pack<T>::K;
   long long pb, pe;
   block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
@@ -399,6 +432,9 @@ __device__ __forceinline__ void fin_apply_state(SysState* st, int& f, const doub
       if (fused) {   // beta = 0 restarts the fused recurrence (classic: k_pupdate does);
         st->beta = cplx<double>{0.0, 0.0};   // the step is taken, so the next apply is
         f &= ~(SF_RESTART | SF_MR);           // an ordinary one
+        st->xc1 = alpha;                      // x_lo += alpha g now (a replacement has
+        st->xc2 = cplx<double>{0.0, 0.0};     // just flushed any deferred term)
+        f &= ~SF_XSKIP;
       }
     }
   } else {
@@ -416,6 +452,24 @@ __device__ __forceinline__ void fin_apply_state(SysState* st, int& f, const doub
         cplx<double> beta = cdiv(alpha * alpha * qtq, rho) - cplx<double>{1.0, 0.0};
         if (!finite(beta.re) || !finite(beta.im)) beta = cplx<double>{0.0, 0.0};
         st->beta = beta;
+        // Deferred solution update: x_lo needs alpha p of every pass, but only at a
+        // replacement.  A pass may leave x_lo alone when the next one can recover that
+        // term: the pass forms p' = r' + beta p, so alpha p = (alpha / beta) (p' - r').
+        // The next pass then adds both terms at once, x_lo += (xa + alpha') p' - xa r'
+        // (xa = alpha / beta), and x_lo is read and written every other pass.  A pending
+        // term is flushed by k_accum before a replacement changes p and r.
+        if (f & SF_XPEND) {
+          st->xc1 = st->xa + alpha;
+          st->xc2 = cplx<double>{-st->xa.re, -st->xa.im};
+          f &= ~(SF_XPEND | SF_XSKIP);
+        } else if (norm2(beta) >= kDeferBeta2) {
+          st->xa = cdiv(alpha, beta);
+          f |= SF_XSKIP | SF_XPEND;
+        } else {
+          st->xc1 = alpha;
+          st->xc2 = cplx<double>{0.0, 0.0};
+          f &= ~SF_XSKIP;
+        }
       }
       st->alpha = alpha;
       f &= ~SF_RESTART;
@@ -506,7 +560,8 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     const double t3[3] = {tot[0], tot[1], tot[2]};
     fin_update_state(st, f, t3, fused);
   } else if constexpr (KIND == FIN_RESID) {
-    // residual replacement / initial residual
+    // residual replacement / initial residual (k_accum has flushed any deferred term)
+    f &= ~(SF_XPEND | SF_XSKIP);
     const double rn2 = tot[0];
     const cplx<double> rho{tot[1], tot[2]};
     st->true_rnorm2 = rn2;
