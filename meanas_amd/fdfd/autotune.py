@@ -9,7 +9,9 @@ the grid, the precision and the material kinds, so it is measured: every candida
 (tile height, slab) is timed with `mx_time_kernels`, i.e. the solver's own iteration
 launched in solve order on the real operator (so the caches hold what they hold in a
 solve), and the fastest apply is kept per (device, shape, precision, material flags), in
-memory and in a JSON cache file.
+memory and in a JSON cache file.  Shapes within 3% of the fastest (timing noise) are
+ranked by a fixed preference (4-row tiles, then the deepest slab), so that near-ties
+resolve the same way on every box.
 
 A system's block partition -- and so the order of its reductions -- depends on that
 system alone (its shape and materials through this table), never on the rest of a
@@ -30,7 +32,12 @@ from .. import _lib
 TILE_ROWS = (2, 4, 8)
 SLABS = (2, 4, 6, 8, 11, 12, 16, 24, 32)
 MAX_PARTIALS = 4096       # per-system reduction limit of the finishing kernel
-CACHE_PATH = os.path.join(os.path.expanduser('~'), '.cache', 'meanas_amd', 'autotune2.json')
+# Shapes within this fraction of the fastest count as tied (timing noise) and are ranked
+# by a fixed preference instead: near-ties would otherwise resolve differently on
+# different boxes, and a system's bits -- even its Krylov iteration count, through the
+# order of its reductions -- depend on its block partition.
+MARGIN = 0.03
+CACHE_PATH = os.path.join(os.path.expanduser('~'), '.cache', 'meanas_amd', 'autotune3.json')
 
 _lock = threading.Lock()
 _memo: dict[str, tuple[int, int]] = {}
@@ -40,6 +47,14 @@ _loaded = False
 def blocks(shape, tile_rows: int, slab: int) -> int:
     nx, ny, nz = shape
     return ((ny + tile_rows - 1) // tile_rows) * ((nz + 63) // 64) * ((nx + slab - 1) // slab)
+
+
+def choose(times: dict[tuple[int, int], float]) -> tuple[int, int]:
+    """Among the shapes within MARGIN of the fastest, the one a fixed preference ranks
+    first: 4-row tiles, then the deepest slab, then the lower tile."""
+    best = min(times.values())
+    near = [c for c, t in times.items() if t <= (1.0 + MARGIN) * best]
+    return min(near, key=lambda c: (c[0] != 4, -c[1], c[0]))
 
 
 def candidates(shape) -> list[tuple[int, int]]:
@@ -109,7 +124,7 @@ def tuned(op, mode: int) -> tuple[int, int]:
         times = measure(op, mode)
         if not times:
             return 0, 0
-        best = min(times, key=times.get)
+        best = choose(times)
         _memo[key] = best
         _save()
         return best

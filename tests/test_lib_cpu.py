@@ -197,3 +197,14 @@ def test_autotuner_candidates_respect_the_partial_limit():
     assert all(autotune.blocks((128, 128, 128), ty, s) <= autotune.MAX_PARTIALS for ty, s in c)
     assert all(s <= 8 or s == autotune.SLABS[0] for _, s in autotune.candidates((8, 48, 48)))
     assert autotune.candidates((1, 2048, 2048)) == []
+
+
+def test_autotuner_resolves_near_ties_by_a_fixed_preference():
+    """Shapes within 3% of the fastest are tied; the tie goes to 4-row tiles and the
+    deepest slab, so timing noise does not change a system's partition between boxes."""
+    from meanas_amd.fdfd import autotune
+    t = {(4, 16): 42.8, (4, 11): 42.3, (2, 16): 43.9, (4, 8): 48.0}
+    assert autotune.choose(t) == (4, 16)
+    t[(4, 11)] = 40.0
+    assert autotune.choose(t) == (4, 11)
+    assert autotune.choose({(2, 8): 10.0, (8, 32): 10.1}) == (8, 32)
