@@ -30,12 +30,16 @@ from .device import DeviceOperator
 
 
 def _field_mask(mask, n3: int, ncell: int) -> numpy.ndarray:
-    """A 3N 0/1 mask from a 3N vector, an (nx, ny, nz) grid mask or a (3, nx, ny, nz) field mask."""
+    """The 3N float64 values of a mask given as a 3N vector, an (nx, ny, nz) grid mask or a
+    (3, nx, ny, nz) field mask.  Values are kept as they are: Q = diag(TF_region) uses
+    them (fdfd/operators.py:392-393), so a fractional or weighted region scales E."""
     m = numpy.asarray(mask)
+    if numpy.iscomplexobj(m):
+        raise ValueError('a mask must be real')
     if m.size == n3:
-        return (m.reshape(-1) != 0).astype(numpy.float64)
+        return numpy.ascontiguousarray(m.reshape(-1), dtype=numpy.float64)
     if m.size == ncell:
-        return numpy.tile((m.reshape(-1) != 0).astype(numpy.float64), 3)
+        return numpy.tile(numpy.asarray(m.reshape(-1), dtype=numpy.float64), 3)
     raise ValueError(f'mask must have N = {ncell} or 3N = {n3} entries')
 
 
@@ -89,8 +93,9 @@ def e_tfsf_source(TF_region, omega: complex, dxes, epsilon, mu=None,
 
 
 def boundary_mask(mask, shape: Sequence[int], periodic_mask_edges: bool = False) -> numpy.ndarray:
-    """3N 0/1 mask of the cells where shifting `mask` by one cell along any axis longer
-    than 1 (either way) changes it; edges mirror unless periodic (operators.py:420-445)."""
+    """3N 0/1 mask of the entries where shifting `mask` by one cell along any axis longer
+    than 1 (either way) changes its value; edges mirror unless periodic
+    (operators.py:420-445: |(S - I) mask| != 0 per component)."""
     n = int(numpy.prod(shape))
     m = _field_mask(mask, 3 * n, n).reshape(3, *shape)
     out = numpy.zeros(m.shape, dtype=bool)

@@ -13,7 +13,7 @@ def rel(a, b):
     return numpy.linalg.norm(a - b) / numpy.linalg.norm(b)
 
 
-def _problem(shape, seed):
+def _problem(shape, seed, weighted=False):
     rng = numpy.random.default_rng(seed)
     dxes = [[1.0 + 0.3 * rng.random(n) for n in shape] for _ in range(2)]
     for ax in range(3):
@@ -25,14 +25,17 @@ def _problem(shape, seed):
     mu = 1.0 + rng.random(n3)
     e = rng.standard_normal(n3) + 1j * rng.standard_normal(n3)
     mask = (rng.random(n3) < 0.5).astype(float)
+    if weighted:   # a fractional region: Q = diag(TF_region) uses the values themselves
+        mask = mask * rng.choice([0.25, 0.5, 1.0], size=n3)
     return rng, dxes, eps, mu, e, mask
 
 
 @pytest.mark.parametrize('shape', [(10, 12, 7), (1, 16, 12), (12, 1, 1)])
 @pytest.mark.parametrize('with_mu', [False, True])
-def test_tfsf_source_matches_oracle(shape, with_mu):
+@pytest.mark.parametrize('weighted', [False, True])
+def test_tfsf_source_matches_oracle(shape, with_mu, weighted):
     from meanas_amd.fdfd import sources
-    rng, dxes, eps, mu, e, mask = _problem(shape, 7)
+    rng, dxes, eps, mu, e, mask = _problem(shape, 7, weighted)
     mu = mu if with_mu else None
     omega = 0.3 - 0.002j
     got = sources.e_tfsf_source(mask, omega, dxes, eps, mu)(e)
@@ -42,9 +45,10 @@ def test_tfsf_source_matches_oracle(shape, with_mu):
 
 @pytest.mark.parametrize('shape', [(10, 12, 7), (12, 1, 1)])
 @pytest.mark.parametrize('periodic', [False, True])
-def test_boundary_source_matches_oracle(shape, periodic):
+@pytest.mark.parametrize('weighted', [False, True])
+def test_boundary_source_matches_oracle(shape, periodic, weighted):
     from meanas_amd.fdfd import sources
-    rng, dxes, eps, mu, e, mask = _problem(shape, 8)
+    rng, dxes, eps, mu, e, mask = _problem(shape, 8, weighted)
     got = sources.e_boundary_source(mask, 0.25, dxes, eps, periodic_mask_edges=periodic)(e)
     want = ref.e_boundary_source(mask, 0.25, dxes, eps, periodic_mask_edges=periodic) @ e
     assert rel(got, want) < 1e-12

@@ -172,7 +172,8 @@ def test_generic_rejects_callable_matrix_solver():
 
 
 @pytest.mark.parametrize('periodic', [False, True])
-def test_boundary_mask_matches_oracle(periodic):
+@pytest.mark.parametrize('weighted', [False, True])
+def test_boundary_mask_matches_oracle(periodic, weighted):
     """Host edge mask of e_boundary_source (fdfd/operators.py:420-445) against the oracle's
     sparse construction: diag(mask) @ e_tfsf_source equals e_boundary_source."""
     from meanas_amd.fdfd.sources import boundary_mask
@@ -181,6 +182,8 @@ def test_boundary_mask_matches_oracle(periodic):
     dxes = [[1.0 + 0.2 * rng.random(n) for n in shape] for _ in range(2)]
     eps = 1.0 + rng.random(3 * 30)
     m = (rng.random(3 * 30) < 0.4).astype(float)
+    if weighted:   # a fractional mask: edges are where the shifted value differs
+        m = m * rng.choice([0.5, 1.0], size=m.size)
     keep = boundary_mask(m, shape, periodic)
     full = ref.e_tfsf_source(m, 0.3, dxes, eps)
     want = ref.e_boundary_source(m, 0.3, dxes, eps, periodic_mask_edges=periodic)
