@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 QMR_ITERS_128 = 6447
 # HBM bytes per launch (MB) from rocprofv3 FETCH_SIZE / WRITE_SIZE counters, scaled by the
 # factors a 1 GiB copy gives (profiles/*_hbm_traffic.md), keyed by kernel, mode, shape
-TRAFFIC_MB = {('update', 'fp64', (128, 128, 128)): 705.0,   # profiles/iteration_kernels_boxes128_hbm_traffic.md
+TRAFFIC_MB = {('update', 'fp64', (128, 128, 128)): 604.4,   # profiles/iteration_kernels_boxes128_hbm_traffic.md
               ('apply', 'fp64', (128, 128, 128)): 231.2}
 
 
