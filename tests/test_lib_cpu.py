@@ -211,3 +211,32 @@ def test_autotuner_resolves_near_ties_by_a_fixed_preference():
     t[(4, 11)] = 40.0
     assert autotune.choose(t) == (4, 11)
     assert autotune.choose({(2, 8): 10.0, (8, 32): 10.1}) == (8, 32)
+
+
+def test_admission_footprint_counts_materials_and_mask():
+    """The server's host-side footprint (computed before anything is allocated) counts the
+    workspace (mx_work_bytes), b, x and the source upload, and grows with complex and
+    anisotropic eps, 1/mu and the PEC/PMC mask, in both precisions."""
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import device_bytes, operator_flags
+    shape = (6, 5, 4)
+    n = 6 * 5 * 4
+    iso = numpy.full(3 * n, 2.0)
+    aniso = numpy.concatenate([numpy.full(n, 2.0), numpy.full(n, 3.0), numpy.full(n, 4.0)])
+    assert operator_flags(shape, iso) == 0
+    assert operator_flags(shape, aniso) == _lib.MX_EPS_ANISO
+    assert operator_flags(shape, iso + 0.1j) == _lib.MX_EPS_COMPLEX
+    pec = numpy.zeros(3 * n)
+    assert operator_flags(shape, iso, pec=pec) == 0          # an all-zero mask is no mask
+    pec[7] = 1
+    assert operator_flags(shape, iso, mu=iso, pec=pec) == _lib.MX_MU | _lib.MX_MASK
+    for mode, real in ((_lib.MX_FP64, 8), (_lib.MX_MIXED, 4)):
+        base = device_bytes(shape, 0, mode)
+        desc = _lib.MxOperator(*shape, 0, 0.0, 0.0, None, None, None, None, None, None, None)
+        work = _lib.lib().mx_work_bytes(ctypes.byref(desc), mode)
+        assert base == sum(shape) * 4 * 24 + n * 12 + 3 * 16 * 3 * n + work
+        assert device_bytes(shape, _lib.MX_EPS_ANISO, mode) - base == 2 * n * 12
+        assert device_bytes(shape, _lib.MX_EPS_COMPLEX, mode) - base == n * 12
+        assert device_bytes(shape, _lib.MX_MU, mode) - base == n * 12
+        assert device_bytes(shape, _lib.MX_MASK, mode) - base == n
+        assert work >= 4 * real * 2 * 3 * n
