@@ -416,6 +416,62 @@ def config5_serving(args, dev, qps, count):
     return out
 
 
+def beyond_cache(args, dev, n=256):
+    """The iteration kernels at n^3 (256^3: each complex128 vector 805 MB, far beyond the
+    256 MB Infinity Cache), timed in solve order with HIP events: HBM rates without the
+    cache's help, on the largest grid of the serving stream."""
+    import torch
+    from meanas_amd import _lib, workloads
+    from meanas_amd.fdfd.device import DeviceOperator, time_kernels
+    w = workloads.boxes_128(n)
+    op = DeviceOperator(w['omega'], w['dxes'], w['epsilon'], device=dev)
+    b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(w['J']).to(dev), coef=-1j * w['omega'])
+    kt = time_kernels(op, b, _lib.MX_FP64, reps=20)
+    a_bytes = op.apply_bytes_per_cell(16) * op.ncell
+    u_bytes = 6 * 16 * op.n3
+    out = dict(workload=f'boxes: {n}^3, eps = 2.085 + boxes, FP64 iteration kernels', tiling=[kt['tile_rows'], kt['slab']],
+               apply=dict(us=kt['apply'], bytes=a_bytes, gbs=a_bytes / kt['apply'] / 1e3, frac=a_bytes / kt['apply'] / 1e3 / HBM_PEAK_GBS),
+               update=dict(us=kt['update'], bytes=u_bytes, gbs=u_bytes / kt['update'] / 1e3, frac=u_bytes / kt['update'] / 1e3 / HBM_PEAK_GBS),
+               finishing_us=kt['fin_apply'],
+               iteration_us=sum(v for k, v in kt.items() if k not in ('slab', 'tile_rows')))
+    log(f'beyond the cache ({n}^3): {out}')
+    del op, b
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_vs_gpu_config1(args, dev):
+    """Config 1 (48^3 plain FDFD) solved to rtol 1e-8 by the oracle on this host (restated
+    CSR + scipy qmr, complex128, to convergence) and by generic() on the GPU."""
+    import torch
+    from oracle import fdfd_ref as ref
+    from meanas_amd import workloads
+    from meanas_amd.fdfd import solvers
+    w = workloads.plain_48()
+    t0 = time.perf_counter()
+    E_cpu = ref.generic(w['omega'], w['dxes'], w['J'], w['epsilon'], matrix_solver_opts=dict(rtol=args.rtol, atol=0.0))
+    t_cpu = time.perf_counter() - t0
+    its = ref.scipy_qmr.last_iterations
+    solvers.generic(w['omega'], w['dxes'], w['J'], w['epsilon'], matrix_solver_opts=dict(rtol=args.rtol))   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E_gpu = solvers.generic(w['omega'], w['dxes'], w['J'], w['epsilon'], matrix_solver_opts=dict(rtol=args.rtol))
+    t_gpu = time.perf_counter() - t0
+    info = dict(solvers.last_info)
+    A0 = ref.e_full(w['omega'], w['dxes'], w['epsilon'])
+    b0 = -1j * w['omega'] * w['J']
+    res = lambda E: float(numpy.linalg.norm(A0 @ E - b0) / numpy.linalg.norm(b0))
+    out = dict(workload='plain_48: 48^3, 10-cell SC-PML, Si core along x, Jy dipole, rtol %g' % args.rtol,
+               cpu=dict(s=t_cpu, solves_per_s=1 / t_cpu, qmr_iterations=its, physical_residual=res(E_cpu),
+                        cores=1, kind='port (oracle: restated CSR e_full + Pl/Pr, scipy qmr)'),
+               gpu=dict(s=t_gpu, solves_per_s=1 / t_gpu, iterations=info['iterations'], physical_residual=res(E_gpu),
+                        note='generic() end to end: host arrays in and out'),
+               field_difference=float(numpy.linalg.norm(E_gpu - E_cpu) / numpy.linalg.norm(E_cpu)),
+               gpu_over_cpu=t_cpu / t_gpu)
+    log(f'config 1 on the CPU and the GPU: {out}')
+    return out
+
+
 def precision_study(op, b, args):
     """Why complex64 Krylov vectors do not pay on this operator: COCG (torch vector ops
     around the library's complex64 / complex128 apply plans) with complex64 rounding in
@@ -696,7 +752,9 @@ def main():
     if not args.no_configs:
         extra['precision_study'] = precision_study(op, b, args)
         torch.cuda.empty_cache()
-        cfg = {'config3_multi_rhs': config3_multi_rhs(args, dev)}
+        extra['beyond_cache_256'] = beyond_cache(args, dev)
+        cfg = {'config1_cpu_and_gpu': cpu_vs_gpu_config1(args, dev)}
+        cfg['config3_multi_rhs'] = config3_multi_rhs(args, dev)
         torch.cuda.empty_cache()
         cfg['config4_uneven'] = {f: config4_uneven(args, dev, f) for f in ('fused', 'classic')}
         torch.cuda.empty_cache()

@@ -57,7 +57,10 @@ int fail(const std::string& msg) {
 
 constexpr int kVecBlock = 256;
 constexpr int kVecElemsPerThread = 8;
-constexpr int kVecMaxBlocks = 2048;
+#ifndef MX_VEC_MAX_BLOCKS
+#define MX_VEC_MAX_BLOCKS 2048
+#endif
+constexpr int kVecMaxBlocks = MX_VEC_MAX_BLOCKS;
 constexpr size_t kAlign = 256;
 constexpr int kMaxPartials = kFinThreads * kFinLoads;   // per-system partials one finishing block sums
 

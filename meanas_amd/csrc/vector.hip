@@ -64,7 +64,10 @@ constexpr int VB = 256;   // threads per vector block
 #define MX_STREAM_XR 1
 #endif
 constexpr bool kStreamXR = MX_STREAM_XR != 0;   // streaming accesses to x_lo and r in k_update
-constexpr int VU = 4;     // 16-byte packs in flight per thread
+#ifndef MX_VU
+#define MX_VU 4
+#endif
+constexpr int VU = MX_VU;   // 16-byte packs in flight per thread
 
 __device__ __forceinline__ bool skip_sys(int f, int skip, int need) {
   return (f & skip) || (need && !(f & need));
