@@ -14,20 +14,24 @@ forward operator applied to conjugated data: E_adj = conj(Pl w) with A w = Pr co
 
 matrix_solver selects the GPU method: 'cocg_fp64' (complex128 COCG, the default) or
 'cocg_mixed' (complex64 iterations with FP64 residual replacement and FP64 solution
-accumulation; on the PML workloads measured here it needs more iterations than FP64
-COCG, because complex64 Krylov vectors lose biorthogonality once ||r|| < ~1e-3 ||b||).  A CSR-based callable, as the
-reference accepts, cannot be honoured by a matrix-free solver and raises TypeError.
+accumulation).  Complex64 rounding in the COCG recurrence makes it diverge on PML
+workloads of 48^3 and up; such a solve stalls and finishes by an FP64 continuation from
+its best iterate, and says so in last_info (fp64_fallback, mixed_iterations).  A
+CSR-based callable, as the reference accepts, cannot be honoured by a matrix-free solver
+and raises TypeError.
 matrix_solver_opts honours scipy qmr's keys: rtol (default 1e-5), atol (default 0),
 maxiter (default 10 * 3N) and x0 (scaled initial guess); as in the reference, passing
 E_guess writes the scaled guess into matrix_solver_opts['x0'] (solvers.py:127-132).
-One key of this build: form ('auto', 'fused' or 'classic', the COCG iteration form of
+One key beyond scipy's: form ('auto', 'fused' or 'classic', the COCG iteration form of
 mx_options; 'auto' is fused for FP64).
 Convergence means ||b - A x|| <= max(rtol ||b||, atol) for the FP64 true residual of
 the scaled system.  Like the reference, a solve that does not converge still returns
 an iterate -- here the best one (lowest FP64 true residual seen), where the reference
-returns qmr's last; the status is logged and kept in `last_info`.  A complex64 solve
-that stalls (near-static omega, where omega^2 eps falls below complex64 resolution of
-the curl-curl term) continues in FP64 from its best iterate.
+returns qmr's last; the status is logged and kept in `last_info`.  A J with a part on
+PEC entries (where e_full has zero rows, so no E matches it) is solved in the
+least-squares sense and reported as 'inconsistent' with its residual floor.  Near-static
+problems (omega^2 eps below complex64 resolution of the curl-curl term) go to FP64 up
+front even when 'cocg_mixed' is asked for.
 """
 from __future__ import annotations
 
