@@ -218,7 +218,8 @@ struct Group { int a_off, na, v_off, nv, s_off, ns, ty; bool simple; };
 size_t cache_budget() {
   static const size_t b = [] {
     const char* v = getenv("MEANAS_AMD_CACHE_MB");
-    return (size_t)(v ? atof(v) : 224.0) * (1u << 20);
+    const double mb = v ? atof(v) : 224.0;
+    return mb > 0.0 ? (size_t)(mb * (1u << 20)) : (size_t)0;
   }();
   return b;
 }
