@@ -172,9 +172,17 @@ int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const v
 /* Solve a batch of systems (one or many right-hand sides, one or many operators of
  * uneven size) in one set of launches per iteration.  Replaces the qmr call of
  * fdfd/solvers.py:62 (scipy sparse/linalg/_isolve/iterative.py:844-1045).
- * stats: nsys entries (host memory). */
+ * stats: nsys entries (host memory).
+ * Each host thread keeps the plan of its last successful call (device tables, block
+ * maps, stream, pinned buffers, captured iteration graph; a few KB plus the graph) and
+ * reuses it when the next call passes the same operators, systems (rtol, atol and
+ * maxiter aside) and options: no allocation, upload or capture then.  Environment
+ * MEANAS_AMD_PLAN_CACHE=0 builds every plan afresh. */
 int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys,
              const mx_options* opt, mx_stats* stats, void* stream);
+/* Number of mx_solve calls of the calling thread that reused a kept plan (diagnostics;
+ * no counterpart in the reference). */
+int64_t mx_solve_plan_reuses(void);
 
 /* ---- fields after the solve ------------------------------------------------------
  * Grid of a post-processing call: the 1D cell widths dxes[0] (E grid) and dxes[1]
@@ -311,7 +319,9 @@ int mx_fdtd_poynting_divergence(const mx_fdtd_grid* g, int32_t prec, const void*
  * between consecutive kernels, and reports the average microseconds per launch:
  * us_out[0] operator apply (COCG epilogue), [1] its finishing kernel, [2] the vector
  * update, [3] its finishing kernel, [4] the direction update of the classic form
- * (a no-op launch for MX_FP64 systems in the fused form).  Fills min(n_out, 5) entries.
+ * (a no-op launch for MX_FP64 systems in the fused form), [5] the finishing update
+ * that fused MX_FP64 solves launch after the apply in place of [1], [2] and [3] (mean
+ * over its two state parities; 0 for other systems).  Fills min(n_out, 6) entries.
  * x and the workspace are overwritten. */
 int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_options* opt, int32_t reps,
                     double* us_out, int32_t n_out, void* stream);

@@ -10,6 +10,7 @@
 #include <new>
 #include <string>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -21,7 +22,8 @@ namespace mx {
 hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
 hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
 hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t,
+                                 int parity);
 hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 // vector.hip
@@ -29,6 +31,7 @@ hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockM
 hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_update_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_update_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_update_fin(const SysDesc*, const BlockMap*, int, int parity, hipStream_t);
 hipError_t launch_accum_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_accum_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_bnorm(const SysDesc*, const BlockMap*, int, hipStream_t);
@@ -58,7 +61,7 @@ int fail(const std::string& msg) {
 constexpr int kVecBlock = 256;
 constexpr int kVecElemsPerThread = 8;
 #ifndef MX_VEC_MAX_BLOCKS
-#define MX_VEC_MAX_BLOCKS 2048
+#define MX_VEC_MAX_BLOCKS 512
 #endif
 constexpr int kVecMaxBlocks = MX_VEC_MAX_BLOCKS;
 constexpr size_t kAlign = 256;
@@ -124,6 +127,14 @@ bool merged_fin_enabled() {
   return !(v && v[0] == '0');
 }
 
+// Fused FP64 chunks of an even length: the post-apply finishing step rides in the
+// update launch (k_update_fin; same results).  MEANAS_AMD_FIN_UPDATE=0 keeps it in a
+// launch of its own.
+bool finishing_update_enabled() {
+  const char* v = getenv("MEANAS_AMD_FIN_UPDATE");
+  return !(v && v[0] == '0');
+}
+
 // Automatic slab depth: enough blocks to fill 256 CUs several times over, but deep slabs
 // to amortise the two extra planes a slab reads.  With `reduce` (kernels that leave
 // per-block partial sums) the block count must also stay within what one finishing
@@ -166,7 +177,8 @@ size_t elem_bytes(int mode) { return mode == MX_MIXED ? 8 : 16; }
 // workspace layout per system: r, p, q, x_lo (inner precision), x_best (FP64), state,
 // partials (apply and reductions), the update's partials
 struct WorkLayout {
-  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, off_partial_vec, total;
+  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, off_partial_vec, off_state_odd,
+      off_partial_vec_odd, total;
 };
 
 WorkLayout work_layout(const mx_operator& op, int mode) {
@@ -182,7 +194,9 @@ WorkLayout work_layout(const mx_operator& op, int mode) {
   w.off_partial = w.off_state + align_up(sizeof(SysState));
   const size_t nbmax = (size_t)std::max(apply_blocks_upper(op), kVecMaxBlocks);
   w.off_partial_vec = w.off_partial + align_up(4 * nbmax * sizeof(double));
-  w.total = w.off_partial_vec + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
+  w.off_state_odd = w.off_partial_vec + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
+  w.off_partial_vec_odd = w.off_state_odd + align_up(sizeof(SysState));
+  w.total = w.off_partial_vec_odd + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
   return w;
 }
 
@@ -409,6 +423,8 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.st = reinterpret_cast<SysState*>(wk + w.off_state);
     d.partial = reinterpret_cast<double*>(wk + w.off_partial);
     d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
+    d.st_odd = reinterpret_cast<SysState*>(wk + w.off_state_odd);
+    d.partial_vec_odd = reinterpret_cast<double*>(wk + w.off_partial_vec_odd);
     B.h_sys[s] = d;
     B.h_op[s] = y.op;
     B.h_ty[s] = ty;
@@ -452,22 +468,30 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 }
 
 // one COCG iteration over every runnable system of the batch (the direction update
-// launch only acts on systems in the classic form; see vector.hip)
-int enqueue_iteration(Batch& B, hipStream_t s) {
+// launch only acts on systems in the classic form; see vector.hip).  `parity` >= 0 picks
+// the finishing-update form (fused FP64 batches, iteration j of an even-length run
+// between check points, parity j & 1): the apply, then k_update_fin, which finishes the
+// previous update and this apply and runs the update, two launches per iteration.
+int enqueue_iteration(Batch& B, hipStream_t s, int parity = -1) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
+    if (parity >= 0) {
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, parity));
+      MX_TRY(launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, parity, s));
+      return 0;
+    }
     if (B.merged_fin) {   // the previous update is finished together with this apply
-      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0));
       MX_TRY(launch_fin(FIN_APPLY_MERGED, B.d_sys, B.d_syslist, B.nsl, s));
       MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
       return 0;
     }
     if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   }
@@ -492,8 +516,11 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
 }
 
 int enqueue_group_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
+  // the finishing update alternates its state between two parities, so the state is
+  // back in SysDesc::st only after an even number of iterations
+  const bool fin_update = B.merged_fin && finishing_update_enabled() && iters % 2 == 0 && check_every % 2 == 0;
   for (int it = 0; it < iters; ++it) {
-    if (enqueue_iteration(B, s)) return -1;
+    if (enqueue_iteration(B, s, fin_update ? (it & 1) : -1)) return -1;
     if ((it + 1) % check_every == 0 || it + 1 == iters) {
       // merged launches: finish the last update, so that the state the replacement and
       // the host read is current
@@ -530,6 +557,122 @@ int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
   B.use_all();
   return rc;
 }
+
+// ---- solve plans ---------------------------------------------------------------
+// A batch's device tables, block maps, stream, events, pinned poll buffers and captured
+// chunk graph depend only on its operators, its systems' vectors and tilings and the
+// options -- not on the stopping rules, which enter the per-call initial state.  The
+// last mx_solve of a thread keeps them, and a call with the same inputs reuses them:
+// no allocation, table upload or graph capture (MEANAS_AMD_PLAN_CACHE=0 builds every
+// batch afresh).  The tables hold the caller's device pointers; a kept plan is only
+// used again by a call that passes the same pointers, which then are valid.
+bool plan_cache_enabled() {
+  const char* v = getenv("MEANAS_AMD_PLAN_CACHE");
+  return !(v && v[0] == '0');
+}
+
+std::string plan_key(const mx_operator* ops, int nops, const mx_system* sys, int nsys, const mx_options& o, int dev) {
+  std::string k;
+  auto put = [&](const void* p, size_t n) { k.append(reinterpret_cast<const char*>(p), n); };
+  put(&dev, sizeof dev);
+  put(&nops, sizeof nops);
+  put(&nsys, sizeof nsys);
+  put(ops, sizeof(mx_operator) * (size_t)nops);
+  for (int i = 0; i < nsys; ++i) {
+    mx_system y = sys[i];
+    y.rtol = 0.0;
+    y.atol = 0.0;
+    y.maxiter = 0;
+    put(&y, sizeof y);
+  }
+  put(&o, sizeof o);
+  const char sw[2] = {(char)merged_fin_enabled(), (char)finishing_update_enabled()};
+  put(sw, sizeof sw);
+  return k;
+}
+
+struct SolvePlan {
+  std::string key;
+  Batch* B = nullptr;
+  bool busy = false;       // in use by a call of this thread (the classic re-solve nests)
+  bool relayout = false;   // its block maps were compacted by that call
+};
+// Never destroyed: at process exit the HIP runtime may already be gone.
+thread_local SolvePlan* tl_plan = nullptr;
+thread_local int64_t tl_plan_reuses = 0;
+
+// The batch one mx_solve call runs on: the thread's kept plan when the inputs match,
+// else a new one, which is kept afterwards if the call succeeds.  An outer call's plan
+// is never touched by a nested call.  A failed call drops the plan it used.
+class PlanUse {
+ public:
+  Batch* B = nullptr;
+
+  int acquire(const mx_operator* ops, int nops, const mx_system* sys, int nsys, const mx_options& o) {
+    if (!tl_plan) tl_plan = new SolvePlan;
+    if (!plan_cache_enabled() || tl_plan->busy) {
+      owned_.reset(new Batch);
+      B = owned_.get();
+      return build_batch(*B, ops, nops, sys, nsys, o);
+    }
+    tl_plan->busy = true;
+    owner_ = true;
+    int dev = 0;
+    if (tl_plan->B) {   // (the arguments are checked by build_batch before any HIP call)
+      MX_TRY(hipGetDevice(&dev));
+      key_ = plan_key(ops, nops, sys, nsys, o, dev);
+    }
+    if (tl_plan->B && tl_plan->key == key_) {
+      B = tl_plan->B;
+      reused_ = true;
+      ++tl_plan_reuses;
+      B->ops = ops;
+      if (tl_plan->relayout) {   // back to every system's blocks, and a new graph
+        tl_plan->relayout = false;
+        if (B->gexec) { MX_TRY(hipGraphExecDestroy(B->gexec)); B->gexec = nullptr; }
+        if (B->graph) { MX_TRY(hipGraphDestroy(B->graph)); B->graph = nullptr; }
+        if (layout_groups(*B, std::vector<char>(nsys, 1))) return -1;
+      }
+      return 0;
+    }
+    drop();
+    owned_.reset(new Batch);
+    B = owned_.get();
+    if (build_batch(*B, ops, nops, sys, nsys, o)) return -1;
+    if (key_.empty()) {
+      MX_TRY(hipGetDevice(&dev));
+      key_ = plan_key(ops, nops, sys, nsys, o, dev);
+    }
+    return 0;
+  }
+  // the call succeeded; `compacted`: its block maps no longer cover every system
+  void keep(bool compacted) { ok_ = true; compacted_ = compacted; }
+
+  ~PlanUse() {
+    if (!owner_) return;
+    tl_plan->busy = false;
+    if (!ok_) {
+      if (reused_) drop();
+      return;
+    }
+    if (!reused_) {
+      tl_plan->B = owned_.release();
+      tl_plan->key = key_;
+    }
+    tl_plan->relayout = compacted_;
+  }
+
+ private:
+  static void drop() {
+    delete tl_plan->B;
+    tl_plan->B = nullptr;
+    tl_plan->key.clear();
+    tl_plan->relayout = false;
+  }
+  std::unique_ptr<Batch> owned_;
+  std::string key_;
+  bool owner_ = false, reused_ = false, ok_ = false, compacted_ = false;
+};
 
 // A finished system that did not converge returns its best iterate (restored by
 // restore_best); its reported residual is that iterate's.
@@ -837,12 +980,21 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
   // The iteration runs as in a solve (each kernel right after the one it follows, so
   // caches hold what they would), with an event between consecutive kernels: kernel k's
   // time is the span from the event before it to the event after it.
-  constexpr int kKernels = 5;
+  // Fused FP64 solves launch the apply and the finishing update (k_update_fin) instead:
+  // two iterations of that form follow (parities 0 and 1, then the finishing of the last
+  // update), timed as kernels 6 and 8.
+  const bool fin_update = fp64 && o.form == MX_FORM_FUSED && merged_fin_enabled() && finishing_update_enabled();
+  const int kKernels = fin_update ? 10 : 5;
   auto launch = [&](int k) -> hipError_t {
     switch (k) {
+      case 5: return launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0);
+      case 6: return launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
+      case 7: return launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 1);
+      case 8: return launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, 1, s);
+      case 9: return launch_fin(FIN_UPDATE, B.d_sys, B.d_syslist, B.nsl, s);
       case 0: return fp64 ? launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s)
                           : launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
-      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
+      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0)
                           : launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
       case 2: return launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s);
       case 3: return fp64 ? launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s)
@@ -866,7 +1018,7 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  double tot[kKernels] = {0, 0, 0, 0, 0};
+  std::vector<double> tot(kKernels, 0.0);
   for (int r = 0; r < reps && e == hipSuccess; ++r)
     for (int k = 0; k < kKernels && e == hipSuccess; ++k) {
       float ms = 0.f;
@@ -874,13 +1026,17 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
       tot[k] += ms;
     }
   if (e != hipSuccess) rc = fail(std::string("mx_time_kernels: ") + hipGetErrorString(e));
-  // output order: apply, its finishing kernel, update, its finishing kernel, direction update
-  const int order[kKernels] = {1, 2, 3, 4, 0};
-  for (int k = 0; k < std::min<int>(n_out, kKernels) && !rc; ++k) us_out[k] = 1e3 * tot[order[k]] / reps;
+  // output order: apply, its finishing kernel, update, its finishing kernel, direction
+  // update, finishing update (mean of its two parities; 0 when the solve does not use it)
+  const int order[5] = {1, 2, 3, 4, 0};
+  for (int k = 0; k < std::min<int>(n_out, 5) && !rc; ++k) us_out[k] = 1e3 * tot[order[k]] / reps;
+  if (n_out > 5 && !rc) us_out[5] = fin_update ? 1e3 * (tot[6] + tot[8]) / (2.0 * reps) : 0.0;
   for (auto& x : ev)
     if (x) (void)hipEventDestroy(x);
   return rc;
 }
+
+int64_t mx_solve_plan_reuses(void) { return tl_plan_reuses; }
 
 int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys, const mx_options* opt,
              mx_stats* stats, void* stream) {
@@ -888,8 +1044,9 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   const mx_options o = defaults(opt);
   std::string why;
   if (!valid_options(o, &why)) return fail(why);
-  Batch B;
-  if (build_batch(B, ops, nops, systems, nsys, o)) return -1;
+  PlanUse plan;
+  if (plan.acquire(ops, nops, systems, nsys, o)) return -1;
+  Batch& B = *plan.B;
   B.classic_launch = o.form != MX_FORM_FUSED;
   B.merged_fin = !B.classic_launch && merged_fin_enabled();
   hipStream_t user = reinterpret_cast<hipStream_t>(stream);
@@ -935,7 +1092,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     MX_TRY(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
     return 0;
   };
-  if (o.use_graph && capture()) return -1;
+  if (o.use_graph && !B.gexec && capture()) return -1;   // a kept plan has its graph
   auto launch_chunk = [&]() -> int {
     if (o.use_graph) {
       MX_TRY(hipGraphLaunch(B.gexec, s));
@@ -959,6 +1116,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   // the unfinished systems only and the chunk is captured again.  (A finished system's
   // blocks would otherwise still be dispatched by every launch, only to exit.)
   int launched_blocks = B.nb_apply0;
+  bool compacted = false;
   auto compact = [&](int slot) -> int {   // returns 1 if rebuilt
     if (nsys == 1) return 0;
     std::vector<char> active(nsys, 0);
@@ -972,6 +1130,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
       MX_TRY(hipMemcpy(&B.h_state[slot][i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
       active[i] = !(B.h_state[slot][i].flags & SF_DONE);
     }
+    compacted = true;
     if (layout_groups(B, active)) return -1;
     launched_blocks = B.nb_apply0;
     if (B.nb_apply0 > 0 && o.use_graph && capture()) return -1;
@@ -1039,6 +1198,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     for (int i = 0; i < nsys; ++i) stats[i] = st_out[i];
   MX_TRY(hipEventRecord(B.ev[0], s));
   MX_TRY(hipStreamWaitEvent(user, B.ev[0], 0));
+  plan.keep(compacted);
   return 0;
 }
 
@@ -1246,6 +1406,8 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   d.st = reinterpret_cast<SysState*>(wk + w.off_state);
   d.partial = reinterpret_cast<double*>(wk + w.off_partial);
   d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
+  d.st_odd = reinterpret_cast<SysState*>(wk + w.off_state_odd);
+  d.partial_vec_odd = reinterpret_cast<double*>(wk + w.off_partial_vec_odd);
   S->h_sys[slot] = d;
   S->h_ops[slot] = *op;
   S->h_ty[slot] = ty;

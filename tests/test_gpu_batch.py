@@ -6,6 +6,7 @@
   * an uneven batch of structures (different grid sizes, PEC blocks): every solution
     meets the physical residual target with the oracle's operator;
   * one finishing launch per fused iteration gives the same bits as two;
+  * a solve plan kept from the previous call gives the same bits as a fresh one;
   * the continuous-batching server: each result is the same alone and in the mix
     (bit for bit), forward and adjoint requests, with admission while others iterate.
 """
@@ -59,8 +60,10 @@ def test_multi_rhs_batch(method):
 
 @pytest.mark.parametrize('case', ['aniso_batch', 'near_static'])
 def test_merged_finishing_equals_separate(case, monkeypatch):
-    """Fused batches finish the update and the next apply in one launch; separate
-    launches (MEANAS_AMD_MERGED_FIN=0) must give the same bits and statistics."""
+    """Fused batches finish the previous update and the apply inside the update launch
+    (k_update_fin, even-length chunks), or in one finishing launch
+    (MEANAS_AMD_FIN_UPDATE=0, or an odd chunk); separate launches
+    (MEANAS_AMD_MERGED_FIN=0) must give the same bits and statistics."""
     import torch
     from meanas_amd import _lib
     from meanas_amd.fdfd.device import DeviceOperator, solve_batch
@@ -80,18 +83,79 @@ def test_merged_finishing_equals_separate(case, monkeypatch):
     op = DeviceOperator(omega, dxes, eps)
     bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega) for J in Js]
     runs = []
-    for merged in ('1', '0'):
+    for merged, fin_update, chunk in (('1', '1', 16), ('1', '0', 16), ('1', '1', 15), ('0', '1', 16)):
         monkeypatch.setenv('MEANAS_AMD_MERGED_FIN', merged)
+        monkeypatch.setenv('MEANAS_AMD_FIN_UPDATE', fin_update)
         xs = [torch.zeros_like(b) for b in bs]
         st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol) for b, x in zip(bs, xs)], mode=_lib.MX_FP64,
-                         options=dict(form='fused'))
+                         options=dict(form='fused', chunk=chunk, check_every=chunk))
         runs.append((xs, st))
-    (xm, sm), (xs_, ss) = runs
-    for a, b in zip(xm, xs_):
-        assert torch.equal(a, b)
-    assert sm == ss
+    (xm, sm) = runs[0]
+    for xs_, ss in runs[1:]:
+        for a, b in zip(xm, xs_):
+            assert torch.equal(a, b)
+        assert sm == ss
     if case == 'aniso_batch':   # the near-static system reaches the iteration cap, identically
         assert all(s['status'] == 'converged' for s in sm), sm
+
+
+def test_kept_plans_equal_fresh_plans(monkeypatch):
+    """mx_solve keeps the last call's tables, maps and captured graph per thread and reuses
+    them when the next call has the same inputs.  Consecutive calls on the same buffers --
+    changed stopping rules, a batch whose maps are compacted as its systems finish, and a
+    fused system continued by a nested classic solve -- must give the same bits and
+    statistics as calls that build every plan afresh, and must reuse the kept plans."""
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    omega, dxes, eps, Js = small_aniso(nrhs=3)
+    op = DeviceOperator(omega, dxes, eps)
+    bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega) for J in Js]
+    xs = [torch.zeros_like(b) for b in bs]
+    works = [torch.empty(op.work_bytes(_lib.MX_FP64), dtype=torch.uint8, device=b.device) for b in bs]
+    # the reference's near-static fixture: its fused solve loses track and finishes in a
+    # nested classic-form solve
+    shape = (5, 5, 5)
+    eps_s = numpy.ones((3, *shape))
+    eps_s[:, 2, 2, 2] = 2.5
+    dxes_s = [[numpy.full(5, 1.5) for _ in range(3)] for _ in range(2)]
+    J_s = numpy.zeros((3, *shape), dtype=complex)
+    J_s[:, 3, 3, 3] = 1 + 1j
+    J_s[:, 1, 1, 1] = 1 - 1j
+    op_s = DeviceOperator(1 / 1500, dxes_s, eps_s.ravel())
+    b_s = op_s.map(_lib.MX_MAP_RHS, torch.from_numpy(J_s.ravel()).cuda(), coef=-1j / 1500)
+    x_s = torch.zeros_like(b_s)
+    work_s = torch.empty(op_s.work_bytes(_lib.MX_FP64), dtype=torch.uint8, device=b_s.device)
+
+    def calls():
+        out = []
+        for rtols in ((1e-4, 1e-7, 1e-9), (1e-4, 1e-7, 1e-9), (1e-9, 1e-9, 1e-9), (1e-6, 1e-8, 1e-5)):
+            for x in xs:
+                x.zero_()
+            st = solve_batch([op], [dict(op=0, b=b, x=x, work=w, rtol=r) for b, x, w, r in zip(bs, xs, works, rtols)],
+                             mode=_lib.MX_FP64, options=dict(form='fused'))
+            out.append(([x.clone() for x in xs], st))
+        for rtol in (1e-11, 1e-11):
+            x_s.zero_()
+            st = solve_batch([op_s], [dict(op=0, b=b_s, x=x_s, work=work_s, rtol=rtol)], mode=_lib.MX_FP64,
+                             options=dict(form='fused'))
+            out.append(([x_s.clone()], st))
+        return out
+
+    monkeypatch.setenv('MEANAS_AMD_PLAN_CACHE', '0')
+    fresh = calls()
+    monkeypatch.setenv('MEANAS_AMD_PLAN_CACHE', '1')
+    before = _lib.lib().mx_solve_plan_reuses()
+    kept = calls()
+    reuses = _lib.lib().mx_solve_plan_reuses() - before
+    # the batch: calls 2-4 reuse the plan of call 1 (call 1 compacts it); the
+    # near-static system: its second call reuses the first call's plan
+    assert reuses >= 4, reuses
+    assert any(s['iterations'] != fresh[0][1][0]['iterations'] for s in fresh[0][1])   # uneven finish
+    for (xa, sa), (xb, sb) in zip(kept, fresh):
+        for a, b in zip(xa, xb):
+            assert torch.equal(a, b)
+        assert sa == sb
 
 
 def test_uneven_structures_with_pec():
