@@ -549,7 +549,7 @@ def main():
 
     import torch
     from meanas_amd import _lib, workloads
-    from meanas_amd.fdfd.device import KERNEL_NAMES, DeviceOperator, solve_batch, time_kernels
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch, time_kernels
 
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
     if args.workload != 'single':
@@ -657,7 +657,6 @@ def main():
     # tiling as the measured solves
     stream = torch.cuda.current_stream()
     kt = time_kernels(op, b, step_mode, reps=args.apply_reps, stream=stream)
-    KERNELS = KERNEL_NAMES
     fp64 = step_mode == _lib.MX_FP64
     elem = 16 if fp64 else 8
     n3 = op.n3
@@ -668,23 +667,17 @@ def main():
         # x_lo, r, p, q read, x_lo, r written
         'update': 6 * elem * n3,
         'pupdate': 0 if fp64 else 3 * elem * n3,                        # fused form: no-op launch
-        # the same update, after summing the apply's and the previous update's partials
-        'update_fin': 6 * elem * n3 if kt['update_fin'] > 0 else 0,
     }
     names = {'apply': f'k_apply<{"double" if fp64 else "float"}, COCG epilogue>',
              'update': f'k_update<{"double" if fp64 else "float"}>{" (fused direction update, x_lo every other launch)" if fp64 else ""}',
-             'pupdate': f'k_pupdate<{"double" if fp64 else "float"}>',
-             'update_fin': 'k_update_fin (finishing step + fused direction update, x_lo every other launch)'}
-    # the kernels a solve launches per iteration: the apply and the finishing update
-    # (fused FP64), or the five separate launches
-    in_solve = ('apply', 'update_fin') if kt['update_fin'] > 0 else ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')
-    it_us = sum(kt[k] for k in in_solve)
-    kern = {k: dict(us=round(kt[k], 2), share=round(kt[k] / it_us, 3) if k in in_solve else None) for k in KERNELS
-            if kt[k] > 0}
+             'pupdate': f'k_pupdate<{"double" if fp64 else "float"}>'}
+    it_us = sum(kt[k] for k in ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate'))
+    kern = {k: dict(us=round(kt[k], 2), share=round(kt[k] / it_us, 3)) for k in
+            ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')}
     for k, v in alg.items():
         if v:
             kern[k].update(bytes=v, gbs=round(v / (kt[k] * 1e-6) / 1e9, 1))
-    dom = max((k for k in in_solve if alg.get(k)), key=lambda k: kt[k])
+    dom = max(('apply', 'update'), key=lambda k: kt[k])
     log(f'iteration kernels (us): {kern}; dominant {dom}')
 
     def roof(k):

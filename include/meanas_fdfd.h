@@ -116,10 +116,14 @@ typedef struct mx_stats {
   int32_t status;        /* MX_CONVERGED, ... */
   int32_t iterations;    /* inner COCG iterations */
   int32_t replacements;  /* FP64 residual replacements (incl. the initial residual) */
-  int32_t flags;         /* raw device state flags */
+  int32_t flags;         /* raw device state flags (MX_FLAG_SWITCHED below) */
   double bnorm;          /* ||b|| */
   double rel_residual;   /* ||b - A x|| / ||b|| in FP64 at exit (scaled system) */
 } mx_stats;
+/* mx_stats.flags bit of a fused-form session system that finished because its fused
+ * recursion lost track (true and recursive residuals parted): continue it from its x in
+ * a classic-form session (mx_solve does this itself). */
+#define MX_FLAG_SWITCHED 8192
 
 int mx_abi_version(void);
 const char* mx_last_error(void);
@@ -319,9 +323,7 @@ int mx_fdtd_poynting_divergence(const mx_fdtd_grid* g, int32_t prec, const void*
  * between consecutive kernels, and reports the average microseconds per launch:
  * us_out[0] operator apply (COCG epilogue), [1] its finishing kernel, [2] the vector
  * update, [3] its finishing kernel, [4] the direction update of the classic form
- * (a no-op launch for MX_FP64 systems in the fused form), [5] the finishing update
- * that fused MX_FP64 solves launch after the apply in place of [1], [2] and [3] (mean
- * over its two state parities; 0 for other systems).  Fills min(n_out, 6) entries.
+ * (a no-op launch for MX_FP64 systems in the fused form).  Fills min(n_out, 5) entries.
  * x and the workspace are overwritten. */
 int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_options* opt, int32_t reps,
                     double* us_out, int32_t n_out, void* stream);
@@ -333,7 +335,10 @@ typedef struct mx_session mx_session;
 mx_session* mx_session_create(int32_t capacity, const mx_options* opt);
 /* admit one system (operator copied into the session); returns the slot or < 0 */
 int mx_session_admit(mx_session* s, const mx_operator* op, const mx_system* sys, void* stream);
-/* run `iterations` COCG iterations on every active slot */
+/* run `iterations` COCG iterations on every active slot.  Fused-form sessions launch
+ * as mx_solve's fused batches do (apply, one finishing kernel, update per iteration):
+ * a slot that loses track finishes with MX_FLAG_SWITCHED instead of switching form in
+ * place. */
 int mx_session_step(mx_session* s, int32_t iterations, void* stream);
 /* host copy of every slot's stats; done[i] = 1 when slot i finished (free to retire) */
 int mx_session_poll(mx_session* s, mx_stats* stats, int32_t* done, void* stream);

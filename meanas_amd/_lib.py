@@ -26,6 +26,7 @@ MX_BREAKDOWN = 2
 MX_NONFINITE = 3
 MX_STALLED = 4
 MX_FORM_AUTO, MX_FORM_FUSED, MX_FORM_CLASSIC = 0, 1, 2
+MX_FLAG_SWITCHED = 8192   # mx_stats.flags: a fused session system that lost track, to continue classic
 MX_REAL32, MX_REAL64 = 0, 1
 MX_FIELD_E, MX_FIELD_H = 0, 1
 MX_ENERGY_DXMUL, MX_ENERGY_HSTEP, MX_ENERGY_ESTEP, MX_ENERGY_H2E, MX_ENERGY_E2H, MX_ENERGY_J = range(6)

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
                                                                  const BlockMap* __restrict__ bmap,
                                                                  int in_slot, int out_slot,
                                                                  int skip_flags, int need_flags,
-                                                                 const void* in_ptr, void* out_ptr, int parity) {
+                                                                 const void* in_ptr, void* out_ptr) {
   constexpr int TZ = 64;
   constexpr int LY = TY + 2, LZ = TZ + 2;
   static_assert(TY >= 2 && 2 * TY + 2 <= 64, "tile rows: the z-halo loads take 2 TY + 2 lanes of one wave");
@@ -113,8 +113,7 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
 
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
-  // the finishing update's iterations keep the state of odd iterations in st_odd
-  const SysState* st = parity ? S.st_odd : S.st;
+  SysState* st = S.st;
   bool mr = false;     // rescue step: accumulate ||q||^2 instead of p^T q, q^T q
   bool keep_p = false;  // residual replacement that keeps the search direction
   if (EPI != EPI_PLAIN) {
@@ -442,14 +441,14 @@ static int prefetch_depth(size_t elem) {
 template <typename T, int EPI, typename RT>
 static hipError_t launch_apply_ty(int ty, bool simple, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
-                                  const void* in_ptr = nullptr, void* out_ptr = nullptr, int parity = 0) {
+                                  const void* in_ptr = nullptr, void* out_ptr = nullptr) {
   const int pd = prefetch_depth(sizeof(T));
   // simple-material kernels exist for prefetch depth 1 only (depth 2 is a tuning override)
   const int sm = simple && pd == 1;
 #define MX_APPLY_CASE(TY_, PD_, SM_)                                                                           \
   case (TY_ * 10 + PD_) * 2 + SM_:                                                                            \
     hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_, SM_ != 0>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys, \
-                       bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr, parity);                            \
+                       bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr);                                    \
     break;
   switch ((ty * 10 + pd) * 2 + sm) {
     MX_APPLY_CASE(2, 1, 0)
@@ -479,10 +478,9 @@ hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>* ops, 
   return launch_apply_ty<float, EPI_COCG, float>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
                                                  SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
-hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s,
-                                 int parity) {
+hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
   return launch_apply_ty<double, EPI_COCG, double>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
-                                                   SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s, nullptr, nullptr, parity);
+                                                   SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
 // residual r = b - A x in FP64; r stored as float (mixed mode) or double (FP64 mode)
 hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {

@@ -235,11 +235,6 @@ struct SysDesc {
   double* partial;            // >= 4 * max(nblk_apply, nblk_vec) doubles
   double* partial_vec;        // >= 3 * nblk_vec doubles: the update's partials, kept apart
                               // so the next apply can run before they are summed
-  // The odd-parity copies for the finishing update (k_update_fin): iteration j of a
-  // captured chunk reads state and update partials of parity j & 1 and writes those of
-  // the other parity, so no block reads what another block of the same launch writes.
-  SysState* st_odd;
-  double* partial_vec_odd;
 };
 
 // Work assignment of one block: system index and block index within that system.

@@ -22,8 +22,7 @@ namespace mx {
 hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
 hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
 hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t,
-                                 int parity);
+hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 // vector.hip
@@ -31,7 +30,6 @@ hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockM
 hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_update_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_update_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_update_fin(const SysDesc*, const BlockMap*, int, int parity, hipStream_t);
 hipError_t launch_accum_f32(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_accum_f64(const SysDesc*, const BlockMap*, int, hipStream_t);
 hipError_t launch_bnorm(const SysDesc*, const BlockMap*, int, hipStream_t);
@@ -42,6 +40,7 @@ hipError_t launch_diag(const AxisCoef<double>*, int, int, int, const void*, void
 }  // namespace mx
 
 using namespace mx;
+static_assert(SF_SWITCHED == MX_FLAG_SWITCHED, "the switched flag is part of the C ABI");
 
 namespace {
 
@@ -61,7 +60,7 @@ int fail(const std::string& msg) {
 constexpr int kVecBlock = 256;
 constexpr int kVecElemsPerThread = 8;
 #ifndef MX_VEC_MAX_BLOCKS
-#define MX_VEC_MAX_BLOCKS 512
+#define MX_VEC_MAX_BLOCKS 2048
 #endif
 constexpr int kVecMaxBlocks = MX_VEC_MAX_BLOCKS;
 constexpr size_t kAlign = 256;
@@ -127,14 +126,6 @@ bool merged_fin_enabled() {
   return !(v && v[0] == '0');
 }
 
-// Fused FP64 chunks of an even length: the post-apply finishing step rides in the
-// update launch (k_update_fin; same results).  MEANAS_AMD_FIN_UPDATE=0 keeps it in a
-// launch of its own.
-bool finishing_update_enabled() {
-  const char* v = getenv("MEANAS_AMD_FIN_UPDATE");
-  return !(v && v[0] == '0');
-}
-
 // Automatic slab depth: enough blocks to fill 256 CUs several times over, but deep slabs
 // to amortise the two extra planes a slab reads.  With `reduce` (kernels that leave
 // per-block partial sums) the block count must also stay within what one finishing
@@ -177,8 +168,7 @@ size_t elem_bytes(int mode) { return mode == MX_MIXED ? 8 : 16; }
 // workspace layout per system: r, p, q, x_lo (inner precision), x_best (FP64), state,
 // partials (apply and reductions), the update's partials
 struct WorkLayout {
-  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, off_partial_vec, off_state_odd,
-      off_partial_vec_odd, total;
+  size_t off_r, off_p, off_q, off_xlo, off_xbest, off_state, off_partial, off_partial_vec, total;
 };
 
 WorkLayout work_layout(const mx_operator& op, int mode) {
@@ -194,9 +184,7 @@ WorkLayout work_layout(const mx_operator& op, int mode) {
   w.off_partial = w.off_state + align_up(sizeof(SysState));
   const size_t nbmax = (size_t)std::max(apply_blocks_upper(op), kVecMaxBlocks);
   w.off_partial_vec = w.off_partial + align_up(4 * nbmax * sizeof(double));
-  w.off_state_odd = w.off_partial_vec + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
-  w.off_partial_vec_odd = w.off_state_odd + align_up(sizeof(SysState));
-  w.total = w.off_partial_vec_odd + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
+  w.total = w.off_partial_vec + align_up(3 * (size_t)kVecMaxBlocks * sizeof(double));
   return w;
 }
 
@@ -423,8 +411,6 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     d.st = reinterpret_cast<SysState*>(wk + w.off_state);
     d.partial = reinterpret_cast<double*>(wk + w.off_partial);
     d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
-    d.st_odd = reinterpret_cast<SysState*>(wk + w.off_state_odd);
-    d.partial_vec_odd = reinterpret_cast<double*>(wk + w.off_partial_vec_odd);
     B.h_sys[s] = d;
     B.h_op[s] = y.op;
     B.h_ty[s] = ty;
@@ -468,30 +454,22 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 }
 
 // one COCG iteration over every runnable system of the batch (the direction update
-// launch only acts on systems in the classic form; see vector.hip).  `parity` >= 0 picks
-// the finishing-update form (fused FP64 batches, iteration j of an even-length run
-// between check points, parity j & 1): the apply, then k_update_fin, which finishes the
-// previous update and this apply and runs the update, two launches per iteration.
-int enqueue_iteration(Batch& B, hipStream_t s, int parity = -1) {
+// launch only acts on systems in the classic form; see vector.hip)
+int enqueue_iteration(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
     MX_TRY(launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
-    if (parity >= 0) {
-      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, parity));
-      MX_TRY(launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, parity, s));
-      return 0;
-    }
     if (B.merged_fin) {   // the previous update is finished together with this apply
-      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0));
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
       MX_TRY(launch_fin(FIN_APPLY_MERGED, B.d_sys, B.d_syslist, B.nsl, s));
       MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
       return 0;
     }
     if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0));
+    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   }
@@ -516,11 +494,8 @@ int enqueue_replacement(Batch& B, hipStream_t s) {
 }
 
 int enqueue_group_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
-  // the finishing update alternates its state between two parities, so the state is
-  // back in SysDesc::st only after an even number of iterations
-  const bool fin_update = B.merged_fin && finishing_update_enabled() && iters % 2 == 0 && check_every % 2 == 0;
   for (int it = 0; it < iters; ++it) {
-    if (enqueue_iteration(B, s, fin_update ? (it & 1) : -1)) return -1;
+    if (enqueue_iteration(B, s)) return -1;
     if ((it + 1) % check_every == 0 || it + 1 == iters) {
       // merged launches: finish the last update, so that the state the replacement and
       // the host read is current
@@ -586,8 +561,8 @@ std::string plan_key(const mx_operator* ops, int nops, const mx_system* sys, int
     put(&y, sizeof y);
   }
   put(&o, sizeof o);
-  const char sw[2] = {(char)merged_fin_enabled(), (char)finishing_update_enabled()};
-  put(sw, sizeof sw);
+  const char sw = (char)merged_fin_enabled();
+  put(&sw, sizeof sw);
   return k;
 }
 
@@ -980,21 +955,12 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
   // The iteration runs as in a solve (each kernel right after the one it follows, so
   // caches hold what they would), with an event between consecutive kernels: kernel k's
   // time is the span from the event before it to the event after it.
-  // Fused FP64 solves launch the apply and the finishing update (k_update_fin) instead:
-  // two iterations of that form follow (parities 0 and 1, then the finishing of the last
-  // update), timed as kernels 6 and 8.
-  const bool fin_update = fp64 && o.form == MX_FORM_FUSED && merged_fin_enabled() && finishing_update_enabled();
-  const int kKernels = fin_update ? 10 : 5;
+  constexpr int kKernels = 5;
   auto launch = [&](int k) -> hipError_t {
     switch (k) {
-      case 5: return launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0);
-      case 6: return launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
-      case 7: return launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 1);
-      case 8: return launch_update_fin(B.d_sys, B.d_bmap_vec, B.nb_vec, 1, s);
-      case 9: return launch_fin(FIN_UPDATE, B.d_sys, B.d_syslist, B.nsl, s);
       case 0: return fp64 ? launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s)
                           : launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
-      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s, 0)
+      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
                           : launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
       case 2: return launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s);
       case 3: return fp64 ? launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s)
@@ -1018,7 +984,7 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
     }
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  std::vector<double> tot(kKernels, 0.0);
+  double tot[kKernels] = {0, 0, 0, 0, 0};
   for (int r = 0; r < reps && e == hipSuccess; ++r)
     for (int k = 0; k < kKernels && e == hipSuccess; ++k) {
       float ms = 0.f;
@@ -1026,11 +992,9 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
       tot[k] += ms;
     }
   if (e != hipSuccess) rc = fail(std::string("mx_time_kernels: ") + hipGetErrorString(e));
-  // output order: apply, its finishing kernel, update, its finishing kernel, direction
-  // update, finishing update (mean of its two parities; 0 when the solve does not use it)
-  const int order[5] = {1, 2, 3, 4, 0};
-  for (int k = 0; k < std::min<int>(n_out, 5) && !rc; ++k) us_out[k] = 1e3 * tot[order[k]] / reps;
-  if (n_out > 5 && !rc) us_out[5] = fin_update ? 1e3 * (tot[6] + tot[8]) / (2.0 * reps) : 0.0;
+  // output order: apply, its finishing kernel, update, its finishing kernel, direction update
+  const int order[kKernels] = {1, 2, 3, 4, 0};
+  for (int k = 0; k < std::min<int>(n_out, kKernels) && !rc; ++k) us_out[k] = 1e3 * tot[order[k]] / reps;
   for (auto& x : ev)
     if (x) (void)hipEventDestroy(x);
   return rc;
@@ -1406,8 +1370,6 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   d.st = reinterpret_cast<SysState*>(wk + w.off_state);
   d.partial = reinterpret_cast<double*>(wk + w.off_partial);
   d.partial_vec = reinterpret_cast<double*>(wk + w.off_partial_vec);
-  d.st_odd = reinterpret_cast<SysState*>(wk + w.off_state_odd);
-  d.partial_vec_odd = reinterpret_cast<double*>(wk + w.off_partial_vec_odd);
   S->h_sys[slot] = d;
   S->h_ops[slot] = *op;
   S->h_ty[slot] = ty;
@@ -1416,7 +1378,9 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   MX_TRY(hipStreamSynchronize(s));
   SysState* st = &S->h_state[slot];
   std::memset(st, 0, sizeof(*st));
-  st->flags = SF_PAUSE | SF_RESTART | (S->o.form == MX_FORM_FUSED ? SF_FUSED : 0);
+  // fused systems that lose track finish (MX_FLAG_SWITCHED) for the host to continue in
+  // the classic form, so every slot stays fused and the two-launch iteration applies
+  st->flags = SF_PAUSE | SF_RESTART | (S->o.form == MX_FORM_FUSED ? SF_FUSED | SF_SWITCH_EXIT : 0);
   st->rtol = y->rtol;
   st->atol = y->atol;
   st->maxiter = std::max(1, y->maxiter);
@@ -1468,6 +1432,8 @@ int mx_session_step(mx_session* S, int32_t iterations, void* stream) {
   if (S->nb_apply == 0 || iterations <= 0) return 0;
   Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec, S->d_syslist, S->nsl, S->ty,
                           false, &S->groups);
+  B.classic_launch = S->o.form != MX_FORM_FUSED;
+  B.merged_fin = !B.classic_launch && merged_fin_enabled();
   const int check = std::min<int>(S->o.check_every, iterations);
   int rc = 0;
   if (S->o.use_graph) {

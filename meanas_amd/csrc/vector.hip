@@ -174,25 +174,21 @@ __global__ __launch_bounds__(VB) void k_pupdate(const OpView<T>* __restrict__ op
 }
 
 // ---- x_lo += alpha p ; r -= alpha q ; [p = r + beta p] ; rho, ||r||^2 ----------------
-// The update pass runs in blocks of VBU threads: the finishing update (k_update_fin)
-// reduces the partial sums of the apply and of the previous update in every block, with
-// the block size and summation order of the finishing kernels (k_fin), so both launch
-// patterns give the same bits.
-constexpr int VBU = kFinThreads;
-
 // kFused: the direction update rides along; x_lo += xc1 p + xc2 r (r before this pass's
 // update; xc1 = alpha, xc2 = 0 unless a deferred term is added), or with kSkipX x_lo is
 // left alone (the fused form's deferred solution update).  Classic: x_lo += alpha p.
-// The pass's rho' = r^T r and ||r||^2 partials go to `partial_out`.
 template <typename T, bool kFused, bool kSkipX = false>
-__device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm, const cplx<T> alpha,
-                                            const cplx<T> beta, const cplx<T> xc1, const cplx<T> xc2,
-                                            double* partial_out) {
+__device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm) {
   __shared__ double s_red[3 * 16];
+  SysState* st = S.st;
   cplx<T>* __restrict__ x = reinterpret_cast<cplx<T>*>(S.vec[VS_XLO]);
   cplx<T>* __restrict__ r = reinterpret_cast<cplx<T>*>(S.vec[VS_R]);
   cplx<T>* __restrict__ p = reinterpret_cast<cplx<T>*>(S.vec[VS_P]);
   const cplx<T>* __restrict__ q = reinterpret_cast<const cplx<T>*>(S.vec[VS_Q]);
+  const cplx<T> alpha = cast<T>(gload<double>(&st->alpha, 0));
+  const cplx<T> beta = kFused ? cast<T>(gload<double>(&st->beta, 0)) : zero<T>();
+  const cplx<T> xc1 = kFused ? cast<T>(gload<double>(&st->xc1, 0)) : alpha;
+  const cplx<T> xc2 = kFused ? cast<T>(gload<double>(&st->xc2, 0)) : zero<T>();
   const long long n = S.n3;
   T rr_re = 0, rr_im = 0, nr = 0;
   auto one = [&](cplx<T>& xv, cplx<T>& rv, cplx<T>& pv, cplx<T> qv) {
@@ -209,27 +205,25 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm,
   constexpr int K = Pack<T>::K;
   long long pb, pe;
   block_range(n / K, bm.local, S.nblk_vec, &pb, &pe);
-  for (long long base = pb + threadIdx.x; base < pe; base += (long long)VBU * VU) {
+  for (Trip t{pb + threadIdx.x, pe}; t.base < pe; t.base += (long long)VB * VU) {
     Pack<T> xv[VU], rv[VU], pv[VU], qv[VU];
 #pragma unroll
     for (int u = 0; u < VU; ++u)
-      if (base + (long long)u * VBU < pe) {
-        const long long i = base + (long long)u * VBU;
-        if constexpr (!kSkipX) xv[u] = ld<T, kStreamXR>(x, i);
-        rv[u] = ld<T, kStreamXR>(r, i);
-        pv[u] = ld<T>(p, i);
-        qv[u] = ld<T>(q, i);
+      if (t.ok(u)) {
+        if constexpr (!kSkipX) xv[u] = ld<T, kStreamXR>(x, t.at(u));
+        rv[u] = ld<T, kStreamXR>(r, t.at(u));
+        pv[u] = ld<T>(p, t.at(u));
+        qv[u] = ld<T>(q, t.at(u));
       }
 #pragma unroll
     for (int u = 0; u < VU; ++u)
-      if (base + (long long)u * VBU < pe) {
-        const long long i = base + (long long)u * VBU;
+      if (t.ok(u)) {
 #pragma unroll
         for (int k = 0; k < K; ++k) one(xv[u].v[k], rv[u].v[k], pv[u].v[k], qv[u].v[k]);
         // x and r are next read one iteration later; p is read by the next kernel
-        if constexpr (!kSkipX) stp<T, kStreamXR>(x, i, xv[u]);
-        stp<T, kStreamXR>(r, i, rv[u]);
-        if constexpr (kFused) stp<T>(p, i, pv[u]);
+        if constexpr (!kSkipX) stp<T, kStreamXR>(x, t.at(u), xv[u]);
+        stp<T, kStreamXR>(r, t.at(u), rv[u]);
+        if constexpr (kFused) stp<T>(p, t.at(u), pv[u]);
       }
   }
   if (n % K && bm.local == S.nblk_vec - 1 && threadIdx.x == 0) {
@@ -241,29 +235,21 @@ __device__ __forceinline__ void update_body(const SysDesc& S, const BlockMap bm,
   }
   double acc[3] = {(double)rr_re, (double)rr_im, (double)nr};
   block_sum<3>(acc, s_red);
-  store_partials<3>(acc, partial_out, bm.local, S.nblk_vec);
+  store_partials<3>(acc, S.partial_vec, bm.local, S.nblk_vec);
 }
 
 template <typename T>
-__device__ __forceinline__ void update_dispatch(const SysDesc& S, const BlockMap bm, int f, const SysState& st,
-                                                double* partial_out) {
-  const cplx<T> alpha = cast<T>(st.alpha);
-  if (sizeof(T) == 8 && (f & SF_FUSED)) {   // the direction update rides along
-    const cplx<T> beta = cast<T>(st.beta), xc1 = cast<T>(st.xc1), xc2 = cast<T>(st.xc2);
-    if (f & SF_XSKIP) update_body<T, true, true>(S, bm, alpha, beta, xc1, xc2, partial_out);
-    else update_body<T, true, false>(S, bm, alpha, beta, xc1, xc2, partial_out);
-  } else {
-    update_body<T, false>(S, bm, alpha, zero<T>(), alpha, zero<T>(), partial_out);
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(VBU) void k_update(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
+__global__ __launch_bounds__(VB) void k_update(const SysDesc* __restrict__ systems, const BlockMap* __restrict__ bmap) {
   const BlockMap bm = bmap[blockIdx.x];
   const SysDesc& S = systems[bm.sys];
   const int f = gload1<int>(&S.st->flags, 0);
   if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
-  update_dispatch<T>(S, bm, f, *S.st, S.partial_vec);
+  if (sizeof(T) == 8 && (f & SF_FUSED)) {   // the direction update rides along
+    if (f & SF_XSKIP) update_body<T, true, true>(S, bm);
+    else update_body<T, true, false>(S, bm);
+  } else {
+    update_body<T, false>(S, bm);
+  }
 }
 
 // ---- x (c128) += x_lo ; x_lo = 0   (paused systems) --------------------------------
@@ -530,46 +516,6 @@ __device__ __forceinline__ void fin_update_state(SysState* st, int& f, const dou
   if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
 }
 
-// ---- the finishing update: k_fin<4> and k_update in one launch ----------------------
-// Fused complex128 iteration j of a captured chunk (an even number of iterations between
-// check points): every block sums the apply's partials and the previous update's in the
-// fixed order of k_fin and advances a private copy of the system's state exactly as
-// k_fin<4> does; block 0 stores that state into the other parity's SysState, and every
-// block then runs its part of the update with the new scalars, leaving its partials in
-// the other parity's array.  Reading parity j & 1 and writing the other, no block reads
-// what another block of the launch writes; at a check point (even j) the state is back
-// in S.st.  A paused or finished system is carried over to the other parity unchanged.
-__global__ __launch_bounds__(VBU) void k_update_fin(const SysDesc* __restrict__ systems,
-                                                   const BlockMap* __restrict__ bmap, int parity) {
-  __shared__ double s_red[8 * 16];
-  __shared__ SysState s_st;
-  const BlockMap bm = bmap[blockIdx.x];
-  const SysDesc& S = systems[bm.sys];
-  const SysState* st_in = parity ? S.st_odd : S.st;
-  SysState* st_out = parity ? S.st : S.st_odd;
-  const double* pv_in = parity ? S.partial_vec_odd : S.partial_vec;
-  double* pv_out = parity ? S.partial_vec : S.partial_vec_odd;
-  int f = gload1<int>(&st_in->flags, 0);
-  if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) {
-    if (bm.local == 0 && threadIdx.x == 0) *st_out = *st_in;
-    return;
-  }
-  const bool pending = (f & SF_UPD_PENDING) != 0;
-  double tu[3], ta[4];
-  final_sum2<3, 4>(tu, pv_in, pending ? S.nblk_vec : 0, ta, S.partial, S.nblk_apply, s_red);
-  if (threadIdx.x == 0) {
-    s_st = *st_in;
-    if (pending) fin_update_state(&s_st, f, tu, true);
-    if (!(f & SF_PAUSE)) fin_apply_state(&s_st, f, ta, true);
-    s_st.flags = f;
-    if (bm.local == 0) *st_out = s_st;
-  }
-  __syncthreads();
-  f = s_st.flags;
-  if (f & SF_PAUSE) return;
-  update_dispatch<double>(S, bm, f, s_st, pv_out);
-}
-
 template <int KIND>
 __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
   __shared__ double s_red[8 * 16];
@@ -680,28 +626,13 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
     hipLaunchKernelGGL(kern, dim3(nb), dim3(VB), 0, s, sys, bmap);                             \
     return hipGetLastError();                                                                  \
   }
+MX_VEC_LAUNCH(launch_update_f32, k_update<float>)
+MX_VEC_LAUNCH(launch_update_f64, k_update<double>)
 MX_VEC_LAUNCH(launch_accum_f32, k_accum<float>)
 MX_VEC_LAUNCH(launch_accum_f64, k_accum<double>)
 MX_VEC_LAUNCH(launch_bnorm, k_bnorm)
 MX_VEC_LAUNCH(launch_save_best, k_save_best)
 #undef MX_VEC_LAUNCH
-
-hipError_t launch_update_f32(const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update<float>, dim3(nb), dim3(VBU), 0, s, sys, bmap);
-  return hipGetLastError();
-}
-hipError_t launch_update_f64(const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update<double>, dim3(nb), dim3(VBU), 0, s, sys, bmap);
-  return hipGetLastError();
-}
-// finishing update of the fused complex128 iteration (iteration parity of the chunk)
-hipError_t launch_update_fin(const SysDesc* sys, const BlockMap* bmap, int nb, int parity, hipStream_t s) {
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update_fin, dim3(nb), dim3(VBU), 0, s, sys, bmap, parity);
-  return hipGetLastError();
-}
 
 // post = 0: the iteration's direction update (classic form); post = 1: after a
 // residual replacement (restart or rescue step of the fused form)

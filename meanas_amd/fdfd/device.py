@@ -365,7 +365,7 @@ def _solve_once(ops, systems, mode, options, stream) -> list[dict]:
     return [st.as_dict() for st in stats]
 
 
-KERNEL_NAMES = ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate', 'update_fin')
+KERNEL_NAMES = ('apply', 'fin_apply', 'update', 'fin_update', 'pupdate')
 
 
 def time_kernels(op: DeviceOperator, b: torch.Tensor, mode: int = _lib.MX_FP64, reps: int = 50,

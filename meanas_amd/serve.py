@@ -15,9 +15,13 @@ b, x and the source upload, and the solver workspace (`mx_work_bytes`).  A reque
 footprint is reserved when its host preparation starts, and that preparation (PML
 tables, material compaction, uploads) overlaps with the iterations already queued.
 
-In mixed precision (complex64 iterations, FP64 residual replacement) a request whose
-complex64 iterations stall or break down continues in an FP64 session from its best
-FP64 iterate, as `solve_batch` does; its result carries fp64_fallback and the summed
+Sessions are kept per lane, a (precision, iteration form) pair.  In mixed precision
+(complex64 iterations, FP64 residual replacement) a request whose complex64 iterations
+stall or break down continues in an FP64 session from its best FP64 iterate, as
+`solve_batch` does; its result carries fp64_fallback and the summed iteration counts.
+FP64 sessions run the fused iteration form (two launches per iteration); a request
+whose fused recursion loses track finishes there with MX_FLAG_SWITCHED and continues
+from its iterate in a classic-form FP64 session, as `mx_solve` does, with the summed
 iteration counts.
 """
 from __future__ import annotations
@@ -76,9 +80,14 @@ class _Slot:
     work: torch.Tensor | None
     footprint: int
     mode: int
+    form: str = 'fused'            # iteration form of its lane: 'fused' or 'classic'
     admitted: float = 0.0
-    prior: dict | None = None      # stats of a mixed-precision phase this FP64 slot continues
+    prior: dict | None = None      # summed stats of the phases this slot continues
     extra: dict = field(default_factory=dict)
+
+    @property
+    def lane(self) -> tuple[int, str]:
+        return self.mode, self.form
 
 
 def request_shape(req: Request) -> tuple[int, int, int]:
@@ -92,7 +101,8 @@ def footprint_bytes(req: Request, mode: int) -> int:
 
 
 class Server:
-    """Continuous-batching solver over `mx_session`s (one per precision in use)."""
+    """Continuous-batching solver over `mx_session`s (one per lane in use: precision and
+    iteration form)."""
 
     def __init__(self, capacity: int = 8, mode: int = _lib.MX_FP64, chunk: int = 16,
                  hbm_budget: int | None = None, device: str = 'cuda', options: dict | None = None):
@@ -109,27 +119,36 @@ class Server:
         if options:
             o.update(options)
         self._o = o
-        self.sessions: dict[int, int] = {}
-        self._session(mode)
-        self.slots: dict[tuple[int, int], _Slot] = {}
+        self.lane = (mode, self._form(mode))
+        self.sessions: dict[tuple[int, str], int] = {}
+        self._session(self.lane)
+        self.slots: dict[tuple[tuple[int, str], int], _Slot] = {}
         self.used = 0                   # bytes reserved by prepared and admitted requests
 
-    def _session(self, mode: int):
-        if mode not in self.sessions:
+    def _form(self, mode: int) -> str:
+        """The iteration form of `mode`'s lane: the options' form where the precision allows
+        it (complex64 iterations use the classic form), fused for FP64 by default."""
+        form = self._o['form']
+        if mode != _lib.MX_FP64:
+            return 'classic'
+        return 'fused' if form == 'auto' else form
+
+    def _session(self, lane: tuple[int, str]):
+        if lane not in self.sessions:
             o = self._o
-            form = o['form'] if (mode == _lib.MX_FP64 or o['form'] != 'fused') else 'auto'
+            mode, form = lane
             opts = _lib.MxOptions(mode, o['tile_rows'], o['slab'], o['chunk'], o['check_every'], o['use_graph'],
                                   o['delta'], o['stall_iters'], o['stall_limit'], _lib.FORMS[form])
             L = _lib.lib()
             h = L.mx_session_create(self.capacity, ctypes.byref(opts))
             if not h:
                 raise _lib.MxError('mx_session_create failed: ' + L.mx_last_error().decode(errors='replace'))
-            self.sessions[mode] = h
-        return self.sessions[mode]
+            self.sessions[lane] = h
+        return self.sessions[lane]
 
     @property
     def handle(self):
-        return self.sessions.get(self.mode)
+        return self.sessions.get(self.lane)
 
     def close(self):
         for h in getattr(self, 'sessions', {}).values():
@@ -140,8 +159,8 @@ class Server:
         self.close()
 
     # ---- admission ---------------------------------------------------------------
-    def active(self, mode: int) -> int:
-        return sum(1 for (m, _) in self.slots if m == mode)
+    def active(self, lane: tuple[int, str]) -> int:
+        return sum(1 for (ln, _) in self.slots if ln == lane)
 
     def fits(self, nbytes: int) -> bool:
         return self.used + nbytes <= self.hbm_budget
@@ -167,7 +186,7 @@ class Server:
         except Exception:
             self.used -= fp
             raise
-        return _Slot(req, op, b, x, work, fp, self.mode, extra=dict(floor2=floor2))
+        return _Slot(req, op, b, x, work, fp, self.mode, self.lane[1], extra=dict(floor2=floor2))
 
     def admit(self, slot: _Slot, now: float) -> int:
         req = slot.req
@@ -178,47 +197,66 @@ class Server:
         ty, slab = tuned(slot.op, slot.mode)
         sysd = _lib.MxSystem(0, int(min(maxiter, 2 ** 31 - 1)), slab, ty, req.rtol, req.atol,
                              slot.b.data_ptr(), slot.x.data_ptr(), slot.work.data_ptr())
-        h = self._session(slot.mode)
+        h = self._session(slot.lane)
         idx = _lib.lib().mx_session_admit(h, ctypes.byref(slot.op.desc), ctypes.byref(sysd), torch_stream_handle())
         if idx < 0:
             raise _lib.MxError('mx_session_admit failed: ' + _lib.lib().mx_last_error().decode(errors='replace'))
         if not slot.prior:
             slot.admitted = now
-        self.slots[(slot.mode, idx)] = slot
+        self.slots[(slot.lane, idx)] = slot
         return idx
 
     def step(self) -> None:
-        for mode, h in self.sessions.items():
-            if self.active(mode):
+        for lane, h in self.sessions.items():
+            if self.active(lane):
                 _lib.check(_lib.lib().mx_session_step(h, self.chunk, None), 'mx_session_step')
 
-    def poll(self, mode: int):
+    def poll(self, lane: tuple[int, str]):
         stats = (_lib.MxStats * self.capacity)()
         done = (ctypes.c_int32 * self.capacity)()
-        _lib.check(_lib.lib().mx_session_poll(self.sessions[mode], stats, done, torch_stream_handle()),
+        _lib.check(_lib.lib().mx_session_poll(self.sessions[lane], stats, done, torch_stream_handle()),
                    'mx_session_poll')
         return stats, done
 
+    def _hand_over(self, slot: _Slot, st: dict, mode: int, form: str) -> _Slot:
+        """The slot, to be continued in lane (mode, form) from its iterate; `st`'s counts
+        join those of the phases before it."""
+        p = slot.prior or dict(iterations=0, replacements=0)
+        slot.prior = dict(p, iterations=p['iterations'] + st['iterations'],
+                          replacements=p['replacements'] + st['replacements'])
+        if slot.mode == _lib.MX_MIXED:
+            slot.prior.update(fp64_fallback=True, mixed_iterations=st['iterations'])
+        elif slot.form == 'fused' and form == 'classic':
+            slot.prior.update(fused_iterations=st['iterations'])
+        slot.work = None
+        self.used -= slot.footprint
+        slot.footprint = footprint_bytes(slot.req, mode)
+        slot.mode, slot.form = mode, form
+        return slot
+
     def finish(self, key, stats: _lib.MxStats, now: float):
-        """Retire a finished slot: a Result, or (mixed precision that stalled) the slot to
-        continue in FP64 from its best iterate (its workspace traded for an FP64 one)."""
-        mode, idx = key
+        """Retire a finished slot: a Result, or the slot to continue in another lane from
+        its iterate (its workspace traded for one of that lane): complex64 iterations that
+        stalled continue in FP64, a fused FP64 recursion that lost track in the classic
+        form."""
+        lane, idx = key
         slot = self.slots.pop(key)
-        _lib.check(_lib.lib().mx_session_retire(self.sessions[mode], idx), 'mx_session_retire')
+        _lib.check(_lib.lib().mx_session_retire(self.sessions[lane], idx), 'mx_session_retire')
         st = stats.as_dict()
-        if mode == _lib.MX_MIXED:
+        if lane[0] == _lib.MX_MIXED and st['status'] in ('stalled', 'breakdown'):
+            return self._hand_over(slot, st, _lib.MX_FP64, self._form(_lib.MX_FP64))
+        if st['flags'] & _lib.MX_FLAG_SWITCHED:
+            return self._hand_over(slot, st, _lib.MX_FP64, 'classic')
+        if lane[0] == _lib.MX_MIXED:
             st.update(fp64_fallback=False, mixed_iterations=st['iterations'])
-            if st['status'] in ('stalled', 'breakdown'):
-                slot.work = None
-                self.used -= slot.footprint
-                slot.footprint = footprint_bytes(slot.req, _lib.MX_FP64)
-                slot.mode = _lib.MX_FP64
-                slot.prior = st
-                return slot
         if slot.prior:
             p = slot.prior
-            st = dict(st, fp64_fallback=True, iterations=st['iterations'] + p['iterations'],
-                      replacements=st['replacements'] + p['replacements'], mixed_iterations=p['iterations'])
+            st = dict(st, iterations=st['iterations'] + p['iterations'],
+                      replacements=st['replacements'] + p['replacements'])
+            if 'fp64_fallback' in p:
+                st.update(fp64_fallback=p['fp64_fallback'], mixed_iterations=p['mixed_iterations'])
+            if 'fused_iterations' in p:   # the fused FP64 phase before the classic one
+                st.update(fused_iterations=p['fused_iterations'])
         if slot.extra.get('floor2', 0.0) > 0:
             st = mark_inconsistent(st, slot.extra['floor2'])
         which = _lib.MX_MAP_E_ADJ if slot.req.adjoint else _lib.MX_MAP_E
@@ -228,7 +266,7 @@ class Server:
 
     def _continue(self, slot: _Slot, now: float) -> None:
         self.used += slot.footprint
-        slot.work = torch.empty(slot.op.work_bytes(_lib.MX_FP64), dtype=torch.uint8, device=self.device)
+        slot.work = torch.empty(slot.op.work_bytes(slot.mode), dtype=torch.uint8, device=self.device)
         self.admit(slot, now)
 
     # ---- the request loop ------------------------------------------------------------
@@ -237,7 +275,7 @@ class Server:
         clock from the start of the run; otherwise every request is available at once."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         ready: collections.deque[_Slot] = collections.deque()
-        cont: collections.deque[_Slot] = collections.deque()      # mixed -> FP64 continuations
+        cont: collections.deque[_Slot] = collections.deque()      # continuations in another lane
         results: list[Result] = []
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0)
@@ -246,30 +284,30 @@ class Server:
             now = clock()
             if self.slots:
                 self.step()                       # queued: overlaps the host work below
-            while cont and self.active(_lib.MX_FP64) < self.capacity and self.fits(cont[0].footprint):
+            while cont and self.active(cont[0].lane) < self.capacity and self.fits(cont[0].footprint):
                 self._continue(cont.popleft(), clock())
             # prepare arrived requests while the GPU iterates; the footprint is checked
             # and reserved before anything is allocated for the request
             while pending and (not realtime or pending[0].arrival <= now) and \
-                    len(ready) + self.active(self.mode) < self.capacity:
+                    len(ready) + self.active(self.lane) < self.capacity:
                 if next_fp is None:
                     next_fp = footprint_bytes(pending[0], self.mode)
                 if not self.fits(next_fp):
                     break
                 ready.append(self.prepare(pending.popleft(), next_fp))
                 next_fp = None
-            for mode in list(self.sessions):
-                if not self.active(mode):
+            for lane in list(self.sessions):
+                if not self.active(lane):
                     continue
-                stats, done = self.poll(mode)
+                stats, done = self.poll(lane)
                 now = clock()
-                for key in [k for k in list(self.slots) if k[0] == mode and done[k[1]] == 1]:
+                for key in [k for k in list(self.slots) if k[0] == lane and done[k[1]] == 1]:
                     out = self.finish(key, stats[key[1]], now)
                     if isinstance(out, Result):
                         results.append(out)
                     else:
                         cont.append(out)
-            while ready and self.active(self.mode) < self.capacity:
+            while ready and self.active(self.lane) < self.capacity:
                 self.admit(ready.popleft(), clock())
             if not self.slots and not ready and not cont and pending:
                 if next_fp is not None and not self.fits(next_fp) and (not realtime or pending[0].arrival <= clock()):
@@ -280,7 +318,7 @@ class Server:
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
             if not self.slots and not ready and cont and not self.fits(cont[0].footprint):
-                raise _lib.MxError(f'FP64 continuation of shape {cont[0].op.shape} does not fit the HBM budget')
+                raise _lib.MxError(f'continuation of shape {cont[0].op.shape} does not fit the HBM budget')
         return results
 
 

@@ -8,7 +8,8 @@
   * one finishing launch per fused iteration gives the same bits as two;
   * a solve plan kept from the previous call gives the same bits as a fresh one;
   * the continuous-batching server: each result is the same alone and in the mix
-    (bit for bit), forward and adjoint requests, with admission while others iterate.
+    (bit for bit), forward and adjoint requests, with admission while others iterate;
+    a fused request that loses track continues in the classic form as mx_solve's does.
 """
 import numpy
 import pytest
@@ -60,10 +61,9 @@ def test_multi_rhs_batch(method):
 
 @pytest.mark.parametrize('case', ['aniso_batch', 'near_static'])
 def test_merged_finishing_equals_separate(case, monkeypatch):
-    """Fused batches finish the previous update and the apply inside the update launch
-    (k_update_fin, even-length chunks), or in one finishing launch
-    (MEANAS_AMD_FIN_UPDATE=0, or an odd chunk); separate launches
-    (MEANAS_AMD_MERGED_FIN=0) must give the same bits and statistics."""
+    """Fused batches finish the update and the next apply in one launch; separate
+    launches (MEANAS_AMD_MERGED_FIN=0) must give the same bits and statistics, with
+    even and odd chunk lengths."""
     import torch
     from meanas_amd import _lib
     from meanas_amd.fdfd.device import DeviceOperator, solve_batch
@@ -83,15 +83,13 @@ def test_merged_finishing_equals_separate(case, monkeypatch):
     op = DeviceOperator(omega, dxes, eps)
     bs = [op.map(_lib.MX_MAP_RHS, torch.from_numpy(J).cuda(), coef=-1j * omega) for J in Js]
     runs = []
-    for merged, fin_update, chunk in (('1', '1', 16), ('1', '0', 16), ('1', '1', 15), ('0', '1', 16)):
+    for merged, chunk in (('1', 16), ('0', 16), ('1', 15), ('0', 15)):
         monkeypatch.setenv('MEANAS_AMD_MERGED_FIN', merged)
-        monkeypatch.setenv('MEANAS_AMD_FIN_UPDATE', fin_update)
         xs = [torch.zeros_like(b) for b in bs]
         st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=rtol) for b, x in zip(bs, xs)], mode=_lib.MX_FP64,
                          options=dict(form='fused', chunk=chunk, check_every=chunk))
         runs.append((xs, st))
-    (xm, sm) = runs[0]
-    for xs_, ss in runs[1:]:
+    for (xm, sm), (xs_, ss) in (runs[:2], runs[2:]):
         for a, b in zip(xm, xs_):
             assert torch.equal(a, b)
         assert sm == ss
@@ -224,6 +222,51 @@ def test_server_alone_equals_mixed(method):
             assert numpy.linalg.norm(Aadj @ z - bb) / numpy.linalg.norm(bb) < 1e-8
         else:
             assert physical_residual(r.omega, r.dxes, r.epsilon, r.J, m.E) < 1e-8
+
+
+def _near_static_request(tag, adjoint=False):
+    """The reference's near-static fixture (meanas/test/test_fdfd.py, omega = 1/1500): its
+    fused FP64 recursion loses track and the solve continues in the classic form."""
+    from meanas_amd.serve import Request
+    shape = (5, 5, 5)
+    eps = numpy.ones((3, *shape))
+    eps[:, 2, 2, 2] = 2.5
+    dxes = [[numpy.full(5, 1.5) for _ in range(3)] for _ in range(2)]
+    J = numpy.zeros((3, *shape), dtype=complex)
+    J[:, 3, 3, 3] = 1 + 1j
+    J[:, 1, 1, 1] = 1 - 1j
+    return Request(omega=1 / 1500, dxes=dxes, J=J.ravel(), epsilon=eps.ravel(), adjoint=adjoint, rtol=1e-11,
+                   maxiter=3000, arrival=0.0, tag=tag)
+
+
+def test_server_continues_switched_systems_in_classic_form():
+    """FP64 sessions run the fused form; a request whose fused recursion loses track
+    finishes there (MX_FLAG_SWITCHED) and continues in a classic-form session.  Its result
+    equals mx_solve's (which continues such systems itself) bit for bit, with the same
+    statistics, alone and in a mix with other requests."""
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    from meanas_amd.serve import Server
+    r = _near_static_request('ns')
+    op = DeviceOperator(r.omega, r.dxes, r.epsilon)
+    b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(r.J).cuda(), coef=-1j * r.omega)
+    x = torch.zeros_like(b)
+    ref_st = solve_batch([op], [dict(op=0, b=b, x=x, rtol=r.rtol, maxiter=r.maxiter)], mode=_lib.MX_FP64,
+                         options=dict(chunk=16))[0]
+    E_ref = op.map(_lib.MX_MAP_E, x).cpu().numpy()
+    srv = Server(capacity=1, mode=_lib.MX_FP64, chunk=16)
+    alone = srv.run([r], realtime=False)[0]
+    srv.close()
+    assert 0 < alone.stats.get('fused_iterations', 0) < alone.stats['iterations'], alone.stats
+    assert numpy.array_equal(alone.E, E_ref)
+    for k in ('status', 'iterations', 'replacements', 'rel_residual'):
+        assert alone.stats[k] == ref_st[k], (k, alone.stats, ref_st)
+    srv = Server(capacity=2, mode=_lib.MX_FP64, chunk=16)
+    mixed = {res.tag: res for res in srv.run([r] + _requests()[:3], realtime=False)}
+    srv.close()
+    assert numpy.array_equal(mixed['ns'].E, alone.E)
+    assert mixed['ns'].stats == alone.stats
 
 
 def test_server_matches_generic():
