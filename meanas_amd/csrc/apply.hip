@@ -239,11 +239,20 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
   };
   auto e_ = [&](int c, int rr, int qq) { return sE[c][rr][qq]; };
 
+  // eps of the next plane.  Simple materials hold one value for the three components in
+  // one register: three copies of one load, carried round the plane loop, made the
+  // compiler wait for that load right where it was issued, before the step's second half
+  // (vmcnt counts in order, so the plane prefetch issued with it waited too).  Measured
+  // at 128^3: COCG apply 43.0 -> 41.5 us, plain complex64 apply 35.5 -> 32.2 us.
   cplx<T> epsn[3];
   auto load_eps = [&](int cell) {
+    if constexpr (SM) {
+      epsn[0] = load_mat<T>(op.eps, false, false, 0, cell, nc);
+    } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      epsn[c] = load_mat<T>(op.eps, eps_cplx, eps_aniso, c, cell, nc);
+      for (int c = 0; c < 3; ++c)
+        epsn[c] = load_mat<T>(op.eps, eps_cplx, eps_aniso, c, cell, nc);
+    }
   };
 
   // ---- prologue: H(ib-1) at the own column (y, z components); plane ib into LDS
@@ -285,7 +294,7 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
     const int cell = i * (int)plane + own_off;
     cplx<T> epsv[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) epsv[c] = epsn[c];
+    for (int c = 0; c < 3; ++c) epsv[c] = epsn[SM ? 0 : c];
     const FaFb<T> cxi = coef_u(cx, i);
     const cplx<T> fa_xm = coef_u(cx, wrapi(i - 1, nx)).fa;
     __syncthreads();   // plane i is in LDS; the previous step's reads of sH are done
