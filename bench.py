@@ -38,7 +38,7 @@ QMR_ITERS_128 = 6447
 # HBM bytes per launch (MB) from rocprofv3 FETCH_SIZE / WRITE_SIZE counters, scaled by the
 # factors a 1 GiB copy gives (profiles/*_hbm_traffic.md), keyed by kernel, mode, shape
 TRAFFIC_MB = {('update', 'fp64', (128, 128, 128)): 604.4,   # profiles/iteration_kernels_boxes128_hbm_traffic.md
-              ('apply', 'fp64', (128, 128, 128)): 231.2}
+              ('apply', 'fp64', (128, 128, 128)): 231.4}
 
 
 def parse():
