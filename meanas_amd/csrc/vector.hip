@@ -516,47 +516,9 @@ __device__ __forceinline__ void fin_update_state(SysState* st, int& f, const dou
   if (st->iters >= st->maxiter) f |= SF_PAUSE | SF_LIMIT;
 }
 
+// thread 0's part of k_fin<KIND>: the scalar state after the partial sums `tot`
 template <int KIND>
-__global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
-  __shared__ double s_red[8 * 16];
-  const SysDesc& S = systems[syslist[blockIdx.x]];
-  SysState* st = S.st;
-  int f = st->flags;
-  if (KIND == FIN_APPLY || KIND == FIN_APPLY_MERGED) {
-    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
-  } else if (KIND == FIN_UPDATE) {
-    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, SF_UPD_PENDING)) return;
-  } else if (KIND == FIN_RESID) {
-    if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
-  } else {
-    if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
-  }
-  const bool fused = (f & SF_FUSED) != 0;
-  if constexpr (KIND == FIN_APPLY_MERGED) {
-    const bool pending = (f & SF_UPD_PENDING) != 0;
-    double tu[3], ta[4];
-    final_sum2<3, 4>(tu, S.partial_vec, pending ? S.nblk_vec : 0, ta, S.partial, S.nblk_apply, s_red);
-    if (threadIdx.x != 0) return;
-    if (pending) fin_update_state(st, f, tu, fused);
-    if (!(f & SF_PAUSE)) fin_apply_state(st, f, ta, fused);
-    st->flags = f;
-    return;
-  }
-  double tot[4] = {0.0, 0.0, 0.0, 0.0};
-  if constexpr (KIND == FIN_BNORM) {
-    double t1[1];
-    final_sum<1>(t1, S.partial, S.nblk_vec, s_red);
-    tot[0] = t1[0];
-  } else if constexpr (KIND == FIN_APPLY) {
-    final_sum<4>(tot, S.partial, S.nblk_apply, s_red);
-  } else {
-    double t3[3];
-    final_sum<3>(t3, KIND == FIN_UPDATE ? S.partial_vec : S.partial, KIND == FIN_UPDATE ? S.nblk_vec : S.nblk_apply,
-                 s_red);
-    tot[0] = t3[0]; tot[1] = t3[1]; tot[2] = t3[2];
-  }
-  if (threadIdx.x != 0) return;
-
+__device__ __forceinline__ void finish_state(SysState* st, int f, const double (&tot)[4], bool fused) {
   if constexpr (KIND == FIN_APPLY) {
     fin_apply_state(st, f, tot, fused);
   } else if constexpr (KIND == FIN_UPDATE) {
@@ -618,6 +580,73 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
   }
   st->flags = f;
 }
+
+template <int KIND>
+__global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
+  __shared__ double s_red[8 * 16];
+  const SysDesc& S = systems[syslist[blockIdx.x]];
+  SysState* const gst = S.st;
+  int f = gst->flags;
+  if (KIND == FIN_APPLY || KIND == FIN_APPLY_MERGED) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  } else if (KIND == FIN_UPDATE) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, SF_UPD_PENDING)) return;
+  } else if (KIND == FIN_RESID) {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
+  } else {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
+  }
+  const bool fused = (f & SF_FUSED) != 0;
+  // The state is read in one batch (one 8-byte word per lane of wave 0), in flight beside
+  // the partial sums, updated by thread 0 in LDS and written back in one batch.  (Updated
+  // in place through the global pointer, thread 0's loads after its own stores to the
+  // state formed a chain of about ten memory round trips.)
+  __shared__ SysState s_st;
+  constexpr int kWords = (int)(sizeof(SysState) / 8);
+  static_assert(sizeof(SysState) % 8 == 0 && kWords <= 64, "state words: one per lane of a wave");
+  const int tid = threadIdx.x;
+  unsigned long long word = 0;
+  if (tid < kWords) word = reinterpret_cast<const unsigned long long*>(gst)[tid];
+  auto stage = [&]() {   // after the partial sums: the state into LDS
+    if (tid < kWords) reinterpret_cast<unsigned long long*>(&s_st)[tid] = word;
+    __syncthreads();
+  };
+  auto write_back = [&]() {
+    __syncthreads();
+    if (tid < kWords) reinterpret_cast<unsigned long long*>(gst)[tid] = reinterpret_cast<unsigned long long*>(&s_st)[tid];
+  };
+  SysState* const st = &s_st;
+  if constexpr (KIND == FIN_APPLY_MERGED) {
+    const bool pending = (f & SF_UPD_PENDING) != 0;
+    double tu[3], ta[4];
+    final_sum2<3, 4>(tu, S.partial_vec, pending ? S.nblk_vec : 0, ta, S.partial, S.nblk_apply, s_red);
+    stage();
+    if (tid == 0) {
+      if (pending) fin_update_state(st, f, tu, fused);
+      if (!(f & SF_PAUSE)) fin_apply_state(st, f, ta, fused);
+      st->flags = f;
+    }
+    write_back();
+    return;
+  }
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (KIND == FIN_BNORM) {
+    double t1[1];
+    final_sum<1>(t1, S.partial, S.nblk_vec, s_red);
+    tot[0] = t1[0];
+  } else if constexpr (KIND == FIN_APPLY) {
+    final_sum<4>(tot, S.partial, S.nblk_apply, s_red);
+  } else {
+    double t3[3];
+    final_sum<3>(t3, KIND == FIN_UPDATE ? S.partial_vec : S.partial, KIND == FIN_UPDATE ? S.nblk_vec : S.nblk_apply,
+                 s_red);
+    tot[0] = t3[0]; tot[1] = t3[1]; tot[2] = t3[2];
+  }
+  stage();
+  if (tid == 0) finish_state<KIND>(st, f, tot, fused);
+  write_back();
+}
+
 
 // ---- host launch helpers ----------------------------------------------------------
 #define MX_VEC_LAUNCH(name, kern)                                                              \
