@@ -1379,7 +1379,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   SysState* st = &S->h_state[slot];
   std::memset(st, 0, sizeof(*st));
   // fused systems that lose track finish (MX_FLAG_SWITCHED) for the host to continue in
-  // the classic form, so every slot stays fused and the two-launch iteration applies
+  // the classic form, so every slot stays fused and the three-launch fused iteration applies
   st->flags = SF_PAUSE | SF_RESTART | (S->o.form == MX_FORM_FUSED ? SF_FUSED | SF_SWITCH_EXIT : 0);
   st->rtol = y->rtol;
   st->atol = y->atol;
