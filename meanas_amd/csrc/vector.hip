@@ -584,53 +584,23 @@ __device__ __forceinline__ void finish_state(SysState* st, int f, const double (
 template <int KIND>
 __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__ systems, const int* __restrict__ syslist) {
   __shared__ double s_red[8 * 16];
+  __shared__ SysState s_st;
   const SysDesc& S = systems[syslist[blockIdx.x]];
   SysState* const gst = S.st;
-  int f = gst->flags;
-  if (KIND == FIN_APPLY || KIND == FIN_APPLY_MERGED) {
-    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
-  } else if (KIND == FIN_UPDATE) {
-    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, SF_UPD_PENDING)) return;
-  } else if (KIND == FIN_RESID) {
-    if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
-  } else {
-    if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
-  }
-  const bool fused = (f & SF_FUSED) != 0;
-  // The state is read in one batch (one 8-byte word per lane of wave 0), in flight beside
-  // the partial sums, updated by thread 0 in LDS and written back in one batch.  (Updated
-  // in place through the global pointer, thread 0's loads after its own stores to the
-  // state formed a chain of about ten memory round trips.)
-  __shared__ SysState s_st;
+  // The state (one 8-byte word per lane of wave 0) and the partial sums are read in one
+  // batch; thread 0 updates the state in LDS and it is written back in one batch.  The
+  // skip rule is applied once both are in: a skipped system's sums are dropped.  (Read
+  // first and updated in place through the global pointer, the flags and thread 0's
+  // loads after its own stores formed a chain of about ten memory round trips.)
   constexpr int kWords = (int)(sizeof(SysState) / 8);
   static_assert(sizeof(SysState) % 8 == 0 && kWords <= 64, "state words: one per lane of a wave");
   const int tid = threadIdx.x;
   unsigned long long word = 0;
   if (tid < kWords) word = reinterpret_cast<const unsigned long long*>(gst)[tid];
-  auto stage = [&]() {   // after the partial sums: the state into LDS
-    if (tid < kWords) reinterpret_cast<unsigned long long*>(&s_st)[tid] = word;
-    __syncthreads();
-  };
-  auto write_back = [&]() {
-    __syncthreads();
-    if (tid < kWords) reinterpret_cast<unsigned long long*>(gst)[tid] = reinterpret_cast<unsigned long long*>(&s_st)[tid];
-  };
-  SysState* const st = &s_st;
+  double tot[4] = {0.0, 0.0, 0.0, 0.0}, tu[3] = {0.0, 0.0, 0.0};
   if constexpr (KIND == FIN_APPLY_MERGED) {
-    const bool pending = (f & SF_UPD_PENDING) != 0;
-    double tu[3], ta[4];
-    final_sum2<3, 4>(tu, S.partial_vec, pending ? S.nblk_vec : 0, ta, S.partial, S.nblk_apply, s_red);
-    stage();
-    if (tid == 0) {
-      if (pending) fin_update_state(st, f, tu, fused);
-      if (!(f & SF_PAUSE)) fin_apply_state(st, f, ta, fused);
-      st->flags = f;
-    }
-    write_back();
-    return;
-  }
-  double tot[4] = {0.0, 0.0, 0.0, 0.0};
-  if constexpr (KIND == FIN_BNORM) {
+    final_sum2<3, 4>(tu, S.partial_vec, S.nblk_vec, tot, S.partial, S.nblk_apply, s_red);
+  } else if constexpr (KIND == FIN_BNORM) {
     double t1[1];
     final_sum<1>(t1, S.partial, S.nblk_vec, s_red);
     tot[0] = t1[0];
@@ -642,9 +612,32 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const SysDesc* __restrict__
                  s_red);
     tot[0] = t3[0]; tot[1] = t3[1]; tot[2] = t3[2];
   }
-  stage();
-  if (tid == 0) finish_state<KIND>(st, f, tot, fused);
-  write_back();
+  if (tid < kWords) reinterpret_cast<unsigned long long*>(&s_st)[tid] = word;
+  __syncthreads();
+  int f = s_st.flags;
+  if (KIND == FIN_APPLY || KIND == FIN_APPLY_MERGED) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, 0)) return;
+  } else if (KIND == FIN_UPDATE) {
+    if (skip_sys(f, SF_PAUSE | SF_DONE | SF_INACTIVE, SF_UPD_PENDING)) return;
+  } else if (KIND == FIN_RESID) {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, SF_PAUSE)) return;
+  } else {
+    if (skip_sys(f, SF_DONE | SF_INACTIVE, 0)) return;
+  }
+  const bool fused = (f & SF_FUSED) != 0;
+  SysState* const st = &s_st;
+  if (tid == 0) {
+    if constexpr (KIND == FIN_APPLY_MERGED) {
+      // the previous update's sums count only while its finishing is pending
+      if (f & SF_UPD_PENDING) fin_update_state(st, f, tu, fused);
+      if (!(f & SF_PAUSE)) fin_apply_state(st, f, tot, fused);
+      st->flags = f;
+    } else {
+      finish_state<KIND>(st, f, tot, fused);
+    }
+  }
+  __syncthreads();
+  if (tid < kWords) reinterpret_cast<unsigned long long*>(gst)[tid] = reinterpret_cast<unsigned long long*>(&s_st)[tid];
 }
 
 
