@@ -86,17 +86,18 @@ __device__ __forceinline__ FaFb<T> coef_u(const AxisCoef<T>* tab, int n) {
 #ifndef MX_APPLY_WAVES64
 #define MX_APPLY_WAVES64 2
 #endif
-// The simple-material kernels (SM) need 166 VGPRs in complex128, so they can hold 3
-// waves per SIMD; at the 128^3 tiling (512 blocks of 4 waves) only 2 are used either way,
-// and deeper occupancy measured no faster (slab sweep in the solve), so the bound stays 2.
+// The material-kind kernels (MK other than MK_GENERAL) need 148-166 VGPRs in
+// complex128, so they can hold 3 waves per SIMD; at the 128^3 tiling (512 blocks of 4
+// waves) only 2 are used either way, and deeper occupancy measured no faster (slab sweep
+// in the solve), so the bound stays 2.
 #ifndef MX_APPLY_WAVES64_SM
 #define MX_APPLY_WAVES64_SM 2
 #endif
-template <typename T, bool SM>
-constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : (SM ? MX_APPLY_WAVES64_SM : MX_APPLY_WAVES64);
+template <typename T, int MK>
+constexpr int kWavesPerSimd = sizeof(T) == 4 ? 4 : (MK != MK_GENERAL ? MX_APPLY_WAVES64_SM : MX_APPLY_WAVES64);
 
-template <typename T, int TY, int EPI, typename RT, int PD, bool SM>
-__global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const OpView<T>* __restrict__ ops,
+template <typename T, int TY, int EPI, typename RT, int PD, int MK>
+__global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, MK>)) void k_apply(const OpView<T>* __restrict__ ops,
                                                                  const SysDesc* __restrict__ systems,
                                                                  const BlockMap* __restrict__ bmap,
                                                                  int in_slot, int out_slot,
@@ -139,12 +140,15 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, SM>)) void k_apply(const
   const bool valid = (j < ny) && (k < nz);
   const int jw = wrapi(j, ny), kw = wrapi(k, nz);
 
-  // SM (simple materials): no PEC/PMC mask, no mu, real isotropic eps, known at compile
-  // time, so that code and its registers drop out of the kernel
-  const bool has_mask = !SM && (op.flags & OPF_MASK) != 0;
-  const bool has_mu = !SM && (op.flags & OPF_MU) != 0;
-  const bool eps_cplx = !SM && (op.flags & OPF_EPS_COMPLEX) != 0;
-  const bool eps_aniso = !SM && (op.flags & OPF_EPS_ANISO) != 0;
+  // The material kind MK fixes the format flags at compile time (the launch picks the
+  // kind of its operators), so the code and registers of what it excludes drop out;
+  // MK_GENERAL reads them from the operator.  SM: real isotropic eps, one value for the
+  // three components.
+  constexpr bool SM = MK == MK_SIMPLE || MK == MK_MASK;
+  const bool has_mask = MK == MK_MASK || (MK == MK_GENERAL && (op.flags & OPF_MASK) != 0);
+  const bool has_mu = MK == MK_GENERAL && (op.flags & OPF_MU) != 0;
+  const bool eps_cplx = MK == MK_GENERAL && (op.flags & OPF_EPS_COMPLEX) != 0;
+  const bool eps_aniso = MK == MK_ANISO || (MK == MK_GENERAL && (op.flags & OPF_EPS_ANISO) != 0);
   const AxisCoef<T>* cx = op.coef;
   const AxisCoef<T>* cy = op.coef + nx;
   const AxisCoef<T>* cz = op.coef + nx + ny;
@@ -448,55 +452,56 @@ static int prefetch_depth(size_t elem) {
 }
 
 template <typename T, int EPI, typename RT>
-static hipError_t launch_apply_ty(int ty, bool simple, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
+static hipError_t launch_apply_ty(int ty, int kind, const OpView<T>* ops, const SysDesc* sys, const BlockMap* bmap,
                                   int nblocks, int in_slot, int out_slot, int skip, int need, hipStream_t s,
                                   const void* in_ptr = nullptr, void* out_ptr = nullptr) {
   const int pd = prefetch_depth(sizeof(T));
-  // simple-material kernels exist for prefetch depth 1 only (depth 2 is a tuning override)
-  const int sm = simple && pd == 1;
-#define MX_APPLY_CASE(TY_, PD_, SM_)                                                                           \
-  case (TY_ * 10 + PD_) * 2 + SM_:                                                                            \
-    hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_, SM_ != 0>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys, \
-                       bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr);                                    \
+  // material-kind kernels exist for prefetch depth 1 only (depth 2 is a tuning override)
+  const int mk = pd == 1 ? kind : MK_GENERAL;
+#define MX_APPLY_CASE(TY_, PD_, MK_)                                                                        \
+  case (TY_ * 10 + PD_) * 4 + MK_:                                                                         \
+    hipLaunchKernelGGL((k_apply<T, TY_, EPI, RT, PD_, MK_>), dim3(nblocks), dim3(64, TY_), 0, s, ops, sys,   \
+                       bmap, in_slot, out_slot, skip, need, in_ptr, out_ptr);                                 \
     break;
-  switch ((ty * 10 + pd) * 2 + sm) {
-    MX_APPLY_CASE(2, 1, 0)
-    MX_APPLY_CASE(4, 1, 0)
-    MX_APPLY_CASE(8, 1, 0)
-    MX_APPLY_CASE(2, 2, 0)
-    MX_APPLY_CASE(4, 2, 0)
-    MX_APPLY_CASE(8, 2, 0)
-    MX_APPLY_CASE(2, 1, 1)
-    MX_APPLY_CASE(4, 1, 1)
-    MX_APPLY_CASE(8, 1, 1)
+#define MX_APPLY_KINDS(TY_)                                                                                 \
+  MX_APPLY_CASE(TY_, 1, MK_GENERAL)                                                                         \
+  MX_APPLY_CASE(TY_, 1, MK_SIMPLE)                                                                          \
+  MX_APPLY_CASE(TY_, 1, MK_ANISO)                                                                           \
+  MX_APPLY_CASE(TY_, 1, MK_MASK)                                                                            \
+  MX_APPLY_CASE(TY_, 2, MK_GENERAL)
+  switch ((ty * 10 + pd) * 4 + mk) {
+    MX_APPLY_KINDS(2)
+    MX_APPLY_KINDS(4)
+    MX_APPLY_KINDS(8)
     default: return hipErrorInvalidValue;
   }
+#undef MX_APPLY_KINDS
 #undef MX_APPLY_CASE
   return hipGetLastError();
 }
 
-hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+hipError_t launch_apply_plain_f32(int ty, int kind, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
                                   const void* x, void* y, hipStream_t s) {
-  return launch_apply_ty<float, EPI_PLAIN, float>(ty, simple, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+  return launch_apply_ty<float, EPI_PLAIN, float>(ty, kind, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
 }
-hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
+hipError_t launch_apply_plain_f64(int ty, int kind, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb,
                                   const void* x, void* y, hipStream_t s) {
-  return launch_apply_ty<double, EPI_PLAIN, double>(ty, simple, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
+  return launch_apply_ty<double, EPI_PLAIN, double>(ty, kind, ops, sys, bmap, nb, 0, 0, 0, 0, s, x, y);
 }
-hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<float, EPI_COCG, float>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
+hipError_t launch_apply_cocg_f32(int ty, int kind, const OpView<float>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<float, EPI_COCG, float>(ty, kind, ops, sys, bmap, nb, VS_P, VS_Q,
                                                  SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
-hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_COCG, double>(ty, simple, ops, sys, bmap, nb, VS_P, VS_Q,
+hipError_t launch_apply_cocg_f64(int ty, int kind, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_COCG, double>(ty, kind, ops, sys, bmap, nb, VS_P, VS_Q,
                                                    SF_PAUSE | SF_DONE | SF_INACTIVE, 0, s);
 }
 // residual r = b - A x in FP64; r stored as float (mixed mode) or double (FP64 mode)
-hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_RESID, float>(ty, simple, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+hipError_t launch_apply_resid_r32(int ty, int kind, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, float>(ty, kind, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
 }
-hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
-  return launch_apply_ty<double, EPI_RESID, double>(ty, simple, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
+hipError_t launch_apply_resid_r64(int ty, int kind, const OpView<double>* ops, const SysDesc* sys, const BlockMap* bmap, int nb, hipStream_t s) {
+  return launch_apply_ty<double, EPI_RESID, double>(ty, kind, ops, sys, bmap, nb, VS_X, VS_R, SF_DONE | SF_INACTIVE, SF_PAUSE, s);
 }
 
 }  // namespace mx

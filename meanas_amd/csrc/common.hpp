@@ -145,6 +145,18 @@ enum : int {
   OPF_MASK        = 32,  // uint8 per cell: bits 0-2 PEC x,y,z ; bits 3-5 PMC x,y,z
 };
 
+// Material kinds.  The apply has one instantiation per kind, in which the code and the
+// registers of what the kind excludes drop out at compile time: SIMPLE = real isotropic
+// eps only; ANISO = real per-component eps; MASK = real isotropic eps with a PEC/PMC
+// mask; GENERAL = anything (complex or per-component eps with a mask, mu).
+enum : int { MK_GENERAL = 0, MK_SIMPLE = 1, MK_ANISO = 2, MK_MASK = 3 };
+__host__ __device__ constexpr int mat_kind(int flags) {
+  return (flags & (OPF_MU | OPF_EPS_COMPLEX)) ? MK_GENERAL
+         : (flags & OPF_MASK) ? ((flags & OPF_EPS_ANISO) ? MK_GENERAL : MK_MASK)
+         : (flags & OPF_EPS_ANISO) ? MK_ANISO
+         : MK_SIMPLE;
+}
+
 // Device view of one operator (one structure) in precision T.
 template <typename T> struct OpView {
   int nx, ny, nz, flags;

@@ -19,12 +19,12 @@
 
 namespace mx {
 // apply.hip
-hipError_t launch_apply_plain_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
-hipError_t launch_apply_plain_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
-hipError_t launch_apply_cocg_f32(int ty, bool simple, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_cocg_f64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_resid_r32(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
-hipError_t launch_apply_resid_r64(int ty, bool simple, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_plain_f32(int ty, int kind, const OpView<float>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_plain_f64(int ty, int kind, const OpView<double>*, const SysDesc*, const BlockMap*, int, const void*, void*, hipStream_t);
+hipError_t launch_apply_cocg_f32(int ty, int kind, const OpView<float>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_cocg_f64(int ty, int kind, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r32(int ty, int kind, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
+hipError_t launch_apply_resid_r64(int ty, int kind, const OpView<double>*, const SysDesc*, const BlockMap*, int, hipStream_t);
 // vector.hip
 hipError_t launch_pupdate_f32(const OpView<float>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
 hipError_t launch_pupdate_f64(const OpView<double>*, const SysDesc*, const BlockMap*, int, int, hipStream_t);
@@ -112,12 +112,8 @@ int refresh_iters(int stall_iters) {
   return stall_iters > 0 ? std::min(stall_iters, kRefreshIters) : kRefreshIters;
 }
 
-// Simple materials: no PEC/PMC mask, no mu, real isotropic eps.  The apply has a
-// kernel specialised for them (that code and its registers drop out).
-bool op_simple(const mx_operator& op) {
-  const OpView<double> v = view_of<double>(op);
-  return !(v.flags & (OPF_MASK | OPF_MU | OPF_EPS_COMPLEX | OPF_EPS_ANISO));
-}
+// Material kind of an operator (common.hpp): the apply kernel its launches use.
+int op_kind(const mx_operator& op) { return mat_kind(view_of<double>(op).flags); }
 
 // One finishing launch per fused iteration (default); MEANAS_AMD_MERGED_FIN=0 launches
 // the post-apply and post-update finishing kernels separately (same results).
@@ -214,7 +210,7 @@ std::vector<BlockMap> xcd_order(const std::vector<BlockMap>& natural) {
 // in a single solve (x_lo and r are streamed past the cache by the update).  A group of
 // many small systems also fills the GPU that one of them would leave idle.  A system's
 // block partition does not depend on its group, so results do not either.
-struct Group { int a_off, na, v_off, nv, s_off, ns, ty; bool simple; };
+struct Group { int a_off, na, v_off, nv, s_off, ns, ty, kind; };
 
 // MALL bytes a launch group may fill (MEANAS_AMD_CACHE_MB overrides; 0: one group)
 size_t cache_budget() {
@@ -268,10 +264,10 @@ struct Batch {
   int graph_chunk = 0;
   bool classic_launch = true;   // enqueue the classic-form direction update every iteration
   bool merged_fin = false;      // every system stays fused: one finishing launch per iteration
-  bool simple = false;          // every operator has simple materials (no mask, no mu, real isotropic eps)
+  int kind = MK_GENERAL;        // material kind of every operator (MK_GENERAL when they differ)
 
   void use_group(const Group& g) {
-    ty = g.ty; simple = g.simple;
+    ty = g.ty; kind = g.kind;
     d_bmap_apply = bm_apply0 + g.a_off; nb_apply = g.na;
     d_bmap_vec = bm_vec0 + g.v_off; nb_vec = g.nv;
     d_syslist = syslist0 + g.s_off; nsl = g.ns;
@@ -298,12 +294,12 @@ struct Member {
   const mx_operator* op;
   int op_id;                // systems with the same op_id share materials
   int ty;                   // apply tile height
-  bool simple;              // simple materials (op_simple)
+  int kind;                 // material kind (op_kind)
   int na, nv;               // apply and vector blocks
 };
 
 // Lay `members` out as consecutive launch groups (see Group): one tile height and one
-// material kind (simple or general) per group, and within those, consecutive members
+// material kind per group, and within those, consecutive members
 // while their hot bytes fit the cache budget.  Each group's maps are XCD-ordered on their
 // own; apply, vec and sl receive the concatenated maps and system lists.
 void make_groups(const std::vector<Member>& members, int mode, std::vector<BlockMap>& apply,
@@ -311,7 +307,7 @@ void make_groups(const std::vector<Member>& members, int mode, std::vector<Block
   const size_t budget = cache_budget();
   apply.clear(); vec.clear(); sl.clear(); groups.clear();
   for (int ty : {2, 4, 8})
-    for (int simple = 1; simple >= 0; --simple) {
+    for (int kind : {MK_SIMPLE, MK_ANISO, MK_MASK, MK_GENERAL}) {
       std::vector<BlockMap> ga, gv;
       std::vector<int> gs, gops;
       size_t bytes = 0;
@@ -324,13 +320,13 @@ void make_groups(const std::vector<Member>& members, int mode, std::vector<Block
         vec.insert(vec.end(), ov.begin(), ov.end());
         sl.insert(sl.end(), gs.begin(), gs.end());
         g.na = (int)oa.size(); g.nv = (int)ov.size(); g.ns = (int)gs.size();
-        g.ty = ty; g.simple = simple != 0;
+        g.ty = ty; g.kind = kind;
         groups.push_back(g);
         ga.clear(); gv.clear(); gs.clear(); gops.clear();
         bytes = 0;
       };
       for (const Member& m : members) {
-        if (m.ty != ty || (int)m.simple != simple) continue;
+        if (m.ty != ty || m.kind != kind) continue;
         const bool new_op = std::find(gops.begin(), gops.end(), m.op_id) == gops.end();
         const size_t add = vec_hot_bytes(*m.op, mode) + (new_op ? mat_hot_bytes(*m.op, mode) : 0);
         if (budget > 0 && !gs.empty() && bytes + add > budget) close();
@@ -350,7 +346,7 @@ int layout_groups(Batch& B, const std::vector<char>& active) {
   std::vector<Member> members;
   for (int s = 0; s < B.nsys; ++s)
     if (active[s])
-      members.push_back({s, &B.ops[B.h_op[s]], B.h_op[s], B.h_ty[s], op_simple(B.ops[B.h_op[s]]),
+      members.push_back({s, &B.ops[B.h_op[s]], B.h_op[s], B.h_ty[s], op_kind(B.ops[B.h_op[s]]),
                          B.h_sys[s].nblk_apply, B.h_sys[s].nblk_vec});
   std::vector<BlockMap> apply, vec;
   std::vector<int> sl;
@@ -435,9 +431,9 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 
   std::vector<OpView<float>> v32(nops);
   std::vector<OpView<double>> v64(nops);
-  B.simple = true;
+  B.kind = nops > 0 ? op_kind(ops[0]) : MK_GENERAL;
   for (int i = 0; i < nops; ++i) {
-    B.simple = B.simple && op_simple(ops[i]);
+    if (op_kind(ops[i]) != B.kind) B.kind = MK_GENERAL;
     v64[i] = view_of<double>(ops[i]);
     if (o.mode == MX_MIXED) v32[i] = view_of<float>(ops[i]);
     else std::memset(&v32[i], 0, sizeof(v32[i]));
@@ -458,18 +454,18 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
 int enqueue_iteration(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_cocg_f32(B.ty, B.kind, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   } else {
     if (B.merged_fin) {   // the previous update is finished together with this apply
-      MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+      MX_TRY(launch_apply_cocg_f64(B.ty, B.kind, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
       MX_TRY(launch_fin(FIN_APPLY_MERGED, B.d_sys, B.d_syslist, B.nsl, s));
       MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
       return 0;
     }
     if (B.classic_launch) MX_TRY(launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s));
-    MX_TRY(launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_cocg_f64(B.ty, B.kind, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
     MX_TRY(launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s));
     MX_TRY(launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
   }
@@ -482,10 +478,10 @@ int enqueue_iteration(Batch& B, hipStream_t s) {
 int enqueue_replacement(Batch& B, hipStream_t s) {
   if (B.mode == MX_MIXED) {
     MX_TRY(launch_accum_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
-    MX_TRY(launch_apply_resid_r32(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_resid_r32(B.ty, B.kind, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
   } else {
     MX_TRY(launch_accum_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
-    MX_TRY(launch_apply_resid_r64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
+    MX_TRY(launch_apply_resid_r64(B.ty, B.kind, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s));
   }
   MX_TRY(launch_fin(FIN_RESID, B.d_sys, B.d_syslist, B.nsl, s));
   MX_TRY(launch_save_best(B.d_sys, B.d_bmap_vec, B.nb_vec, s));
@@ -716,7 +712,7 @@ size_t mx_work_bytes(const mx_operator* op, int32_t mode) {
 struct mx_plan {
   int mode, ty, nb;
   char* d_mem;
-  bool simple;
+  int kind;
 };
 
 mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, int32_t slab) {
@@ -739,7 +735,7 @@ mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, 
   const size_t s_op = align_up(std::max(sizeof(OpView<float>), sizeof(OpView<double>)));
   const size_t s_sys = align_up(sizeof(SysDesc));
   const size_t s_bm = align_up(sizeof(BlockMap) * bm.size());
-  mx_plan* P = new (std::nothrow) mx_plan{mode, ty, d.nblk_apply, nullptr, op_simple(*op)};
+  mx_plan* P = new (std::nothrow) mx_plan{mode, ty, d.nblk_apply, nullptr, op_kind(*op)};
   if (!P) { fail("out of host memory"); return nullptr; }
   hipError_t e = hipMalloc(&P->d_mem, s_op + s_sys + s_bm);
   if (e != hipSuccess) { fail(std::string("hipMalloc: ") + hipGetErrorString(e)); delete P; return nullptr; }
@@ -764,9 +760,9 @@ int mx_plan_apply(mx_plan* P, const void* x, void* y, void* stream) {
   const BlockMap* bm = reinterpret_cast<const BlockMap*>(P->d_mem + s_op + s_sys);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (P->mode == MX_MIXED)
-    MX_TRY(launch_apply_plain_f32(P->ty, P->simple, reinterpret_cast<const OpView<float>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+    MX_TRY(launch_apply_plain_f32(P->ty, P->kind, reinterpret_cast<const OpView<float>*>(P->d_mem), sys, bm, P->nb, x, y, s));
   else
-    MX_TRY(launch_apply_plain_f64(P->ty, P->simple, reinterpret_cast<const OpView<double>*>(P->d_mem), sys, bm, P->nb, x, y, s));
+    MX_TRY(launch_apply_plain_f64(P->ty, P->kind, reinterpret_cast<const OpView<double>*>(P->d_mem), sys, bm, P->nb, x, y, s));
   return 0;
 }
 
@@ -857,7 +853,7 @@ int resid_run(const mx_operator& op, SysDesc d, char* mem, double* norm2_out, hi
   if (e == hipSuccess) e = hipMemcpyAsync(bmd, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_one, &zero_idx, sizeof(int), hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
-    e = launch_apply_resid_r64(kAutoTileRows, op_simple(op), reinterpret_cast<const OpView<double>*>(mem),
+    e = launch_apply_resid_r64(kAutoTileRows, op_kind(op), reinterpret_cast<const OpView<double>*>(mem),
                                reinterpret_cast<const SysDesc*>(p_sys), bmd, d.nblk_apply, s);
   if (e == hipSuccess) e = launch_fin(FIN_RESID, reinterpret_cast<const SysDesc*>(p_sys), d_one, 1, s);
   if (e == hipSuccess) e = hipMemcpyAsync(&st, d.st, sizeof(st), hipMemcpyDeviceToHost, s);
@@ -960,8 +956,8 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
     switch (k) {
       case 0: return fp64 ? launch_pupdate_f64(B.d_ops64, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s)
                           : launch_pupdate_f32(B.d_ops32, B.d_sys, B.d_bmap_vec, B.nb_vec, 0, s);
-      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.simple, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
-                          : launch_apply_cocg_f32(B.ty, B.simple, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
+      case 1: return fp64 ? launch_apply_cocg_f64(B.ty, B.kind, B.d_ops64, B.d_sys, B.d_bmap_apply, B.nb_apply, s)
+                          : launch_apply_cocg_f32(B.ty, B.kind, B.d_ops32, B.d_sys, B.d_bmap_apply, B.nb_apply, s);
       case 2: return launch_fin(FIN_APPLY, B.d_sys, B.d_syslist, B.nsl, s);
       case 3: return fp64 ? launch_update_f64(B.d_sys, B.d_bmap_vec, B.nb_vec, s)
                           : launch_update_f32(B.d_sys, B.d_bmap_vec, B.nb_vec, s);
@@ -1244,7 +1240,7 @@ int ensure_staging(mx_session* S, size_t need) {
 // maps of the admitted slot (one group of tile height ty), or with `groups` the launch
 // groups of every active slot.
 Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const BlockMap* bm_vec, int nb_vec,
-                    const int* syslist, int nsl, int ty, bool simple, const std::vector<Group>* groups = nullptr) {
+                    const int* syslist, int nsl, int ty, int kind, const std::vector<Group>* groups = nullptr) {
   Batch B;
   B.mode = S->o.mode;
   B.d_ops32 = S->d_ops32;
@@ -1259,7 +1255,7 @@ Batch session_batch(mx_session* S, const BlockMap* bm_apply, int nb_apply, const
   if (groups) B.groups = *groups;
   B.use_all();
   B.ty = ty;
-  B.simple = simple;
+  B.kind = kind;
   return B;
 }
 
@@ -1276,7 +1272,7 @@ int session_rebuild(mx_session* S) {
   std::vector<Member> members;
   for (int i = 0; i < S->capacity; ++i)
     if (S->active[i])
-      members.push_back({i, &S->h_ops[i], i, S->h_ty[i], op_simple(S->h_ops[i]), S->h_sys[i].nblk_apply,
+      members.push_back({i, &S->h_ops[i], i, S->h_ty[i], op_kind(S->h_ops[i]), S->h_sys[i].nblk_apply,
                          S->h_sys[i].nblk_vec});
   std::vector<BlockMap> a, v;
   std::vector<int> sl;
@@ -1411,7 +1407,7 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   MX_TRY(hipMemcpy(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
   int* adm_sl = S->d_syslist + S->capacity;
   MX_TRY(hipMemcpy(adm_sl, &slot, sizeof(int), hipMemcpyHostToDevice));
-  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1, ty, op_simple(*op));
+  Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1, ty, op_kind(*op));
   int rc = 0;
   if (launch_zero_slot(S->o.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) rc = fail("zero x_lo");
   if (!rc && launch_bnorm(B.d_sys, B.d_bmap_vec, B.nb_vec, s) != hipSuccess) rc = fail("bnorm");
@@ -1431,7 +1427,7 @@ int mx_session_step(mx_session* S, int32_t iterations, void* stream) {
   if (S->dirty && session_rebuild(S)) return -1;
   if (S->nb_apply == 0 || iterations <= 0) return 0;
   Batch B = session_batch(S, S->d_bm, S->nb_apply, S->d_bm + S->nb_apply, S->nb_vec, S->d_syslist, S->nsl, S->ty,
-                          false, &S->groups);
+                          MK_GENERAL, &S->groups);   // each launch group carries its own kind
   B.classic_launch = S->o.form != MX_FORM_FUSED;
   B.merged_fin = !B.classic_launch && merged_fin_enabled();
   const int check = std::min<int>(S->o.check_every, iterations);

@@ -60,13 +60,19 @@ def rel(a, b):
     return numpy.linalg.norm(a - b) / numpy.linalg.norm(b)
 
 
+# One variant per material kind of the kernel (real isotropic eps; real per-component eps;
+# real isotropic eps with PEC/PMC masks; the general kernel twice)
+VARIANTS = dict(plain_iso=dict(eps_kind='iso_real'), aniso_real=dict(eps_kind='aniso_real'),
+                iso_masks=dict(eps_kind='iso_real', masks=True), aniso_complex=dict(eps_kind='aniso_complex'),
+                mu_masks=dict(eps_kind='aniso_real', mu=True, masks=True))
+
+
 @pytest.mark.parametrize('shape', SHAPES)
-@pytest.mark.parametrize('variant', ['plain_iso', 'aniso_complex', 'mu_masks'])
+@pytest.mark.parametrize('variant', list(VARIANTS))
 @pytest.mark.parametrize('mode', ['c64', 'c128'])
 def test_apply_matches_oracle(shape, variant, mode):
     from meanas_amd.fdfd.device import DeviceOperator
-    kw = dict(plain_iso=dict(eps_kind='iso_real'), aniso_complex=dict(eps_kind='aniso_complex'),
-              mu_masks=dict(eps_kind='aniso_real', mu=True, masks=True))[variant]
+    kw = VARIANTS[variant]
     omega, dxes, eps, mu, pec, pmc = make_case(shape, 1 + len(variant) + sum(shape), **kw)
     op = DeviceOperator(omega, dxes, eps, mu=mu, pec=pec, pmc=pmc)
     g = numpy.random.default_rng(7)
@@ -84,11 +90,14 @@ def test_apply_matches_oracle(shape, variant, mode):
 
 @pytest.mark.parametrize('tile_rows', [2, 4, 8])
 @pytest.mark.parametrize('slab', [1, 3, 16])
-def test_apply_tiling_invariance(tile_rows, slab):
-    """Every tile height / slab depth gives the same complex128 result as the oracle."""
+@pytest.mark.parametrize('variant', ['plain_iso', 'aniso_real', 'iso_masks', 'mu_masks'])
+def test_apply_tiling_invariance(tile_rows, slab, variant):
+    """Every tile height / slab depth gives the same complex128 result as the oracle, for
+    every material kind."""
     from meanas_amd.fdfd.device import DeviceOperator
     shape = (19, 21, 75)
-    omega, dxes, eps, mu, pec, pmc = make_case(shape, 3, eps_kind='aniso_complex', mu=True, masks=True)
+    kw = dict(VARIANTS[variant], eps_kind='aniso_complex') if variant == 'mu_masks' else VARIANTS[variant]
+    omega, dxes, eps, mu, pec, pmc = make_case(shape, 3, **kw)
     op = DeviceOperator(omega, dxes, eps, mu=mu, pec=pec, pmc=pmc)
     g = numpy.random.default_rng(8)
     x = g.standard_normal(op.n3) + 1j * g.standard_normal(op.n3)
