@@ -156,6 +156,46 @@ def test_kept_plans_equal_fresh_plans(monkeypatch):
         assert sa == sb
 
 
+def test_kept_plans_are_per_thread():
+    """Two host threads solving different systems at the same time, twice each (the
+    second call reuses each thread's kept plan), give the bits of the same solves run one
+    after the other."""
+    import threading
+    import torch
+    from meanas_amd import _lib
+    from meanas_amd.fdfd.device import DeviceOperator, solve_batch
+    cases = []
+    for shape, seed in (((24, 20, 22), 7), ((20, 26, 18), 8)):
+        omega, dxes, eps, Js = small_aniso(shape=shape, nrhs=1, seed=seed)
+        op = DeviceOperator(omega, dxes, eps)
+        b = op.map(_lib.MX_MAP_RHS, torch.from_numpy(Js[0]).cuda(), coef=-1j * omega)
+        work = torch.empty(op.work_bytes(_lib.MX_FP64), dtype=torch.uint8, device=b.device)
+        cases.append((op, b, work))
+
+    def run(case, x, out, key):
+        op, b, work = case
+        for rtol in (1e-6, 1e-9):
+            x.zero_()
+            st = solve_batch([op], [dict(op=0, b=b, x=x, work=work, rtol=rtol)], mode=_lib.MX_FP64)[0]
+            out[(key, rtol)] = (x.clone(), st)
+        torch.cuda.synchronize()
+
+    seq = {}
+    for k, case in enumerate(cases):
+        run(case, torch.zeros_like(case[1]), seq, k)
+    par = {}
+    threads = [threading.Thread(target=run, args=(case, torch.zeros_like(case[1]), par, k))
+               for k, case in enumerate(cases)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert set(par) == set(seq)
+    for key in seq:
+        assert torch.equal(par[key][0], seq[key][0]), key
+        assert par[key][1] == seq[key][1], key
+
+
 def test_uneven_structures_with_pec():
     import torch
     from meanas_amd import _lib, workloads
