@@ -187,6 +187,10 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
 /* Number of mx_solve calls of the calling thread that reused a kept plan (diagnostics;
  * no counterpart in the reference). */
 int64_t mx_solve_plan_reuses(void);
+/* Free the calling thread's kept plan (its device tables, stream, pinned buffers and
+ * graph); no counterpart in the reference.  A thread that ends keeps its plan's memory
+ * until the process ends unless it calls this first. */
+int mx_solve_plan_release(void);
 
 /* ---- fields after the solve ------------------------------------------------------
  * Grid of a post-processing call: the 1D cell widths dxes[0] (E grid) and dxes[1]

@@ -127,6 +127,7 @@ PROTOTYPES = {
     'mx_solve': (ctypes.c_int, [ctypes.POINTER(MxOperator), ctypes.c_int32, ctypes.POINTER(MxSystem),
                                 ctypes.c_int32, ctypes.POINTER(MxOptions), ctypes.POINTER(MxStats), c_void_p]),
     'mx_solve_plan_reuses': (ctypes.c_int64, []),
+    'mx_solve_plan_release': (ctypes.c_int, []),
     'mx_m2j': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p, c_void_p,
                               c_void_p]),
     'mx_e2h': (ctypes.c_int, [ctypes.POINTER(MxGrid), ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,

@@ -998,6 +998,16 @@ int mx_time_kernels(const mx_operator* op, const mx_system* sys, const mx_option
 
 int64_t mx_solve_plan_reuses(void) { return tl_plan_reuses; }
 
+int mx_solve_plan_release(void) {
+  if (!tl_plan) return 0;
+  if (tl_plan->busy) return fail("mx_solve_plan_release: the plan is in use by a solve on this thread");
+  delete tl_plan->B;
+  tl_plan->B = nullptr;
+  tl_plan->key.clear();
+  tl_plan->relayout = false;
+  return 0;
+}
+
 int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys, const mx_options* opt,
              mx_stats* stats, void* stream) {
   if (!ops || nops < 1 || !systems || nsys < 1) return fail("mx_solve: empty batch");

@@ -149,6 +149,12 @@ def test_kept_plans_equal_fresh_plans(monkeypatch):
     # the batch: calls 2-4 reuse the plan of call 1 (call 1 compacts it); the
     # near-static system: its second call reuses the first call's plan
     assert reuses >= 4, reuses
+    # a released plan is not reused
+    _lib.check(_lib.lib().mx_solve_plan_release(), 'mx_solve_plan_release')
+    before = _lib.lib().mx_solve_plan_reuses()
+    x_s.zero_()
+    solve_batch([op_s], [dict(op=0, b=b_s, x=x_s, work=work_s, rtol=1e-11)], mode=_lib.MX_FP64, options=dict(form='fused'))
+    assert _lib.lib().mx_solve_plan_reuses() == before
     assert any(s['iterations'] != fresh[0][1][0]['iterations'] for s in fresh[0][1])   # uneven finish
     for (xa, sa), (xb, sb) in zip(kept, fresh):
         for a, b in zip(xa, xb):
@@ -180,11 +186,15 @@ def test_kept_plans_are_per_thread():
             out[(key, rtol)] = (x.clone(), st)
         torch.cuda.synchronize()
 
+    def run_and_release(case, x, out, key):
+        run(case, x, out, key)
+        _lib.check(_lib.lib().mx_solve_plan_release(), 'mx_solve_plan_release')   # the thread ends
+
     seq = {}
     for k, case in enumerate(cases):
         run(case, torch.zeros_like(case[1]), seq, k)
     par = {}
-    threads = [threading.Thread(target=run, args=(case, torch.zeros_like(case[1]), par, k))
+    threads = [threading.Thread(target=run_and_release, args=(case, torch.zeros_like(case[1]), par, k))
                for k, case in enumerate(cases)]
     for t in threads:
         t.start()
