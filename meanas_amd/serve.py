@@ -94,6 +94,35 @@ def request_shape(req: Request) -> tuple[int, int, int]:
     return tuple(int(numpy.size(d)) for d in req.dxes[0])
 
 
+def hand_over_prior(prior: dict | None, st: dict, lane: tuple[int, str], to_form: str) -> dict:
+    """Counts of the phases a continued request has run: `prior` (None before its first
+    hand-over) joined with the phase `st` that ran in `lane`, now handed to a lane of
+    form `to_form`.  A complex64 phase marks fp64_fallback and mixed_iterations; a fused
+    FP64 phase handed to the classic form, fused_iterations."""
+    p = prior or dict(iterations=0, replacements=0)
+    out = dict(p, iterations=p['iterations'] + st['iterations'], replacements=p['replacements'] + st['replacements'])
+    mode, form = lane
+    if mode == _lib.MX_MIXED:
+        out.update(fp64_fallback=True, mixed_iterations=st['iterations'])
+    elif form == 'fused' and to_form == 'classic':
+        out.update(fused_iterations=st['iterations'])
+    return out
+
+
+def merge_prior(st: dict, prior: dict | None, lane: tuple[int, str]) -> dict:
+    """A request's final statistics: its last phase `st`, run in `lane`, with the counts of
+    the phases before it."""
+    if lane[0] == _lib.MX_MIXED:   # finished by its complex64 iterations
+        st = dict(st, fp64_fallback=False, mixed_iterations=st['iterations'])
+    if prior:
+        st = dict(st, iterations=st['iterations'] + prior['iterations'],
+                  replacements=st['replacements'] + prior['replacements'])
+        for k in ('fp64_fallback', 'mixed_iterations', 'fused_iterations'):
+            if k in prior:
+                st[k] = prior[k]
+    return st
+
+
 def footprint_bytes(req: Request, mode: int) -> int:
     """HBM bytes one admitted request holds, from its shape and material formats (host only)."""
     shape = request_shape(req)
@@ -221,13 +250,7 @@ class Server:
     def _hand_over(self, slot: _Slot, st: dict, mode: int, form: str) -> _Slot:
         """The slot, to be continued in lane (mode, form) from its iterate; `st`'s counts
         join those of the phases before it."""
-        p = slot.prior or dict(iterations=0, replacements=0)
-        slot.prior = dict(p, iterations=p['iterations'] + st['iterations'],
-                          replacements=p['replacements'] + st['replacements'])
-        if slot.mode == _lib.MX_MIXED:
-            slot.prior.update(fp64_fallback=True, mixed_iterations=st['iterations'])
-        elif slot.form == 'fused' and form == 'classic':
-            slot.prior.update(fused_iterations=st['iterations'])
+        slot.prior = hand_over_prior(slot.prior, st, slot.lane, form)
         slot.work = None
         self.used -= slot.footprint
         slot.footprint = footprint_bytes(slot.req, mode)
@@ -247,16 +270,7 @@ class Server:
             return self._hand_over(slot, st, _lib.MX_FP64, self._form(_lib.MX_FP64))
         if st['flags'] & _lib.MX_FLAG_SWITCHED:
             return self._hand_over(slot, st, _lib.MX_FP64, 'classic')
-        if lane[0] == _lib.MX_MIXED:
-            st.update(fp64_fallback=False, mixed_iterations=st['iterations'])
-        if slot.prior:
-            p = slot.prior
-            st = dict(st, iterations=st['iterations'] + p['iterations'],
-                      replacements=st['replacements'] + p['replacements'])
-            if 'fp64_fallback' in p:
-                st.update(fp64_fallback=p['fp64_fallback'], mixed_iterations=p['mixed_iterations'])
-            if 'fused_iterations' in p:   # the fused FP64 phase before the classic one
-                st.update(fused_iterations=p['fused_iterations'])
+        st = merge_prior(st, slot.prior, lane)
         if slot.extra.get('floor2', 0.0) > 0:
             st = mark_inconsistent(st, slot.extra['floor2'])
         which = _lib.MX_MAP_E_ADJ if slot.req.adjoint else _lib.MX_MAP_E

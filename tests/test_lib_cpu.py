@@ -240,3 +240,33 @@ def test_admission_footprint_counts_materials_and_mask():
         assert device_bytes(shape, _lib.MX_MU, mode) - base == n * 12
         assert device_bytes(shape, _lib.MX_MASK, mode) - base == n
         assert work >= 4 * real * 2 * 3 * n
+
+
+def test_server_phase_counts_across_hand_overs():
+    """A request the server continues in another lane reports the sum of its phases'
+    iterations and replacements, and which phases ran: complex64 -> fused FP64 (switched)
+    -> classic FP64 gives fp64_fallback, mixed_iterations and fused_iterations; a request
+    that stays in one lane reports its own counts unchanged."""
+    from meanas_amd import _lib
+    from meanas_amd.serve import hand_over_prior, merge_prior
+    mixed, fused, classic = (_lib.MX_MIXED, 'fused'), (_lib.MX_FP64, 'fused'), (_lib.MX_FP64, 'classic')
+    ph = [dict(iterations=70, replacements=2, flags=0), dict(iterations=300, replacements=4, flags=0),
+          dict(iterations=55, replacements=1, flags=0)]
+    p = hand_over_prior(None, ph[0], mixed, 'fused')
+    assert p == dict(iterations=70, replacements=2, fp64_fallback=True, mixed_iterations=70)
+    p = hand_over_prior(p, ph[1], fused, 'classic')
+    assert p == dict(iterations=370, replacements=6, fp64_fallback=True, mixed_iterations=70,
+                     fused_iterations=300)
+    st = merge_prior(dict(ph[2]), p, classic)
+    assert (st['iterations'], st['replacements']) == (425, 7)
+    assert (st['fp64_fallback'], st['mixed_iterations'], st['fused_iterations']) == (True, 70, 300)
+    # complex64 to the end, and FP64 fused to the end
+    st = merge_prior(dict(ph[0]), None, mixed)
+    assert st == dict(ph[0], fp64_fallback=False, mixed_iterations=70)
+    assert merge_prior(dict(ph[1]), None, fused) == ph[1]
+    # FP64 fused switched to classic without a complex64 phase
+    st = merge_prior(dict(ph[2]), hand_over_prior(None, ph[1], fused, 'classic'), classic)
+    assert st == dict(ph[2], iterations=355, replacements=5, fused_iterations=300)
+    # a classic FP64 continuation of a complex64 phase (fused form disabled)
+    st = merge_prior(dict(ph[2]), hand_over_prior(None, ph[0], mixed, 'classic'), classic)
+    assert st == dict(ph[2], iterations=125, replacements=3, fp64_fallback=True, mixed_iterations=70)
