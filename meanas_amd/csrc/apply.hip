@@ -259,6 +259,15 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, MK>)) void k_apply(const
     }
   };
 
+  // Output stores (plain and COCG applies) go through one bounded resource per component
+  // (kDropOffset for lanes outside the grid): the same three stores on every path, so the
+  // wait for the next plane's prefetch in the H phase never waits for them.
+  T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
+  const unsigned comp_bytes = (unsigned)nc * (unsigned)sizeof(cplx<T>);
+  __amdgpu_buffer_rsrc_t yrs[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) yrs[c] = bounded_rsrc(reinterpret_cast<char*>(yout) + (size_t)c * comp_bytes, comp_bytes);
+
   // ---- prologue: H(ib-1) at the own column (y, z components); plane ib into LDS
   cplx<T> hpy, hpz;
   {
@@ -282,12 +291,17 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, MK>)) void k_apply(const
     fetch(RA, wrapi(ib + 1, nx));   // plane ib+1 (H(ie-1) needs plane ie, and ib < ie)
     if constexpr (PD == 2)
       if (ib + 2 <= ie) fetch(RB, wrapi(ib + 2, nx));
+    // three dropped stores, so that the loop is entered as every later step is: with the
+    // prefetch followed by three stores
+    if constexpr (EPI != EPI_RESID) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bstore<T>(yrs[c], kDropOffset, zero<T>());
+    }
   }
 
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const cplx<T> w2 = op.w2;
   const bool diag_real = !eps_cplx && w2.im == T(0);
-  T* __restrict__ yout = reinterpret_cast<T*>(out_ptr ? out_ptr : S.vec[out_slot]);
   const double* __restrict__ bvec = reinterpret_cast<const double*>(S.vec[VS_B]);
   RT* __restrict__ rout = reinterpret_cast<RT*>(S.vec[out_slot]);
   RT* __restrict__ pdir = reinterpret_cast<RT*>(S.vec[VS_P]);
@@ -380,6 +394,11 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, MK>)) void k_apply(const
         if ((mown >> c) & 1) out[c] = zero<T>();
       }
     }
+    if constexpr (EPI != EPI_RESID) {
+      const unsigned off = valid ? (unsigned)cell * (unsigned)sizeof(cplx<T>) : kDropOffset;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) bstore<T>(yrs[c], off, out[c]);
+    }
     if (valid) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -399,19 +418,16 @@ __global__ __launch_bounds__(64 * TY, (kWavesPerSimd<T, MK>)) void k_apply(const
           acc[0] += norm2(rr);
           acc[1] += rd.re * rd.re - rd.im * rd.im;
           acc[2] += 2.0 * rd.re * rd.im;
-        } else {
-          gstore<T>(yout, idx, out[c]);
-          if constexpr (EPI == EPI_COCG) {
-            if (!mr) {
-              const cplx<T> pq = ec[c] * out[c];
-              const cplx<T> qq = out[c] * out[c];
-              acc[0] += (double)pq.re;
-              acc[1] += (double)pq.im;
-              acc[2] += (double)qq.re;
-              acc[3] += (double)qq.im;
-            } else {
-              acc[2] += norm2(cast<double>(out[c]));
-            }
+        } else if constexpr (EPI == EPI_COCG) {
+          if (!mr) {
+            const cplx<T> pq = ec[c] * out[c];
+            const cplx<T> qq = out[c] * out[c];
+            acc[0] += (double)pq.re;
+            acc[1] += (double)pq.im;
+            acc[2] += (double)qq.re;
+            acc[3] += (double)qq.im;
+          } else {
+            acc[2] += norm2(cast<double>(out[c]));
           }
         }
       }
