@@ -77,6 +77,29 @@ def test_oversized_cross_section_is_refused_on_the_host(shape):
     assert b'apply tiles' in L.mx_last_error(), L.mx_last_error()
 
 
+def test_oversized_grid_is_refused_on_the_host():
+    """A grid whose complex128 field reaches 4 GiB (448^3 cells) is refused before any
+    device call by every entry point that runs the operator apply: the kernels address a
+    field component with 32-bit byte offsets, and their bounded output stores assume a
+    component below 2 GiB (the pointers below are never dereferenced)."""
+    from meanas_amd import _lib
+    L = _lib.lib()
+    fake = 0x1000
+    op = _lib.MxOperator(448, 448, 448, 0, 1.0, 0.0, fake, fake, fake, fake, None, None, None)
+    norm = ctypes.c_double(0.0)
+    assert L.mx_residual(ctypes.byref(op), fake, fake, fake, ctypes.byref(norm), None) != 0
+    assert b'too large' in L.mx_last_error(), L.mx_last_error()
+    assert L.mx_residual_noalloc(ctypes.byref(op), fake, fake, fake, ctypes.byref(norm), None) != 0
+    assert b'too large' in L.mx_last_error(), L.mx_last_error()
+    assert not L.mx_plan_create(ctypes.byref(op), _lib.MX_FP64, 4, 16)
+    assert b'too large' in L.mx_last_error(), L.mx_last_error()
+    sysd = _lib.MxSystem(0, 10, 0, 0, 1e-8, 0.0, fake, fake, fake)
+    opts = _lib.MxOptions(_lib.MX_FP64, 0, 0, 0, 0, 1, 0.0, 0, 0, 0, 0)
+    stats = (_lib.MxStats * 1)()
+    assert L.mx_solve(ctypes.byref(op), 1, ctypes.byref(sysd), 1, ctypes.byref(opts), stats, None) != 0
+    assert b'too large' in L.mx_last_error(), L.mx_last_error()
+
+
 def test_conducting_boundary_checks_arguments_on_the_host():
     """mx_fdtd_conducting_boundary refuses bad arguments before any device call (each
     call below fails a check; the buffer is never handed to a kernel)."""
