@@ -339,10 +339,13 @@ typedef struct mx_session mx_session;
 mx_session* mx_session_create(int32_t capacity, const mx_options* opt);
 /* admit one system (operator copied into the session); returns the slot or < 0 */
 int mx_session_admit(mx_session* s, const mx_operator* op, const mx_system* sys, void* stream);
-/* run `iterations` COCG iterations on every active slot.  Fused-form sessions launch
- * as mx_solve's fused batches do (apply, one finishing kernel, update per iteration):
- * a slot that loses track finishes with MX_FLAG_SWITCHED instead of switching form in
- * place. */
+/* run `iterations` COCG iterations on every active slot.  Slots run in launch groups
+ * (by tile height, material kind and Infinity Cache budget), one group after the other;
+ * a group of smaller systems repeats its `iterations` up to 16 times, so each group gets
+ * about the same time per step (a slot's results do not depend on it).  Fused-form
+ * sessions launch as mx_solve's fused batches do (apply, one finishing kernel, update
+ * per iteration): a slot that loses track finishes with MX_FLAG_SWITCHED instead of
+ * switching form in place. */
 int mx_session_step(mx_session* s, int32_t iterations, void* stream);
 /* host copy of every slot's stats; done[i] = 1 when slot i finished (free to retire) */
 int mx_session_poll(mx_session* s, mx_stats* stats, int32_t* done, void* stream);

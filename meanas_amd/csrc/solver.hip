@@ -210,7 +210,8 @@ std::vector<BlockMap> xcd_order(const std::vector<BlockMap>& natural) {
 // in a single solve (x_lo and r are streamed past the cache by the update).  A group of
 // many small systems also fills the GPU that one of them would leave idle.  A system's
 // block partition does not depend on its group, so results do not either.
-struct Group { int a_off, na, v_off, nv, s_off, ns, ty, kind; };
+// reps: times a chunk repeats this group's iterations (sessions: equal time per group)
+struct Group { int a_off, na, v_off, nv, s_off, ns, ty, kind, reps = 1; };
 
 // MALL bytes a launch group may fill (MEANAS_AMD_CACHE_MB overrides; 0: one group)
 size_t cache_budget() {
@@ -515,14 +516,16 @@ int enqueue_replacement_groups(Batch& B, hipStream_t s) {
   return rc;
 }
 
-// `iters` iterations of every launch group, one group after the other (a batch view
-// without groups -- sessions -- is one group)
+// `iters` iterations of every launch group (`reps` times that for a group with reps > 1),
+// one group after the other (a batch view without groups is one group).  A repetition
+// is the same chunk with the same check points, so a system's iterates do not depend on
+// how often its group repeats.
 int enqueue_chunk(Batch& B, hipStream_t s, int iters, int check_every) {
   if (B.groups.empty()) return enqueue_group_chunk(B, s, iters, check_every);
   int rc = 0;
   for (const Group& g : B.groups) {
     B.use_group(g);
-    rc = enqueue_group_chunk(B, s, iters, check_every);
+    for (int r = 0; r < g.reps && !rc; ++r) rc = enqueue_group_chunk(B, s, iters, check_every);
     if (rc) break;
   }
   B.use_all();
@@ -1278,6 +1281,27 @@ void release_view(Batch& B) {   // the view owns nothing
   B.h_state[0] = B.h_state[1] = nullptr;
 }
 
+// Equal time per launch group in a session step.  Groups run one after the other, so
+// with the same iteration count a group of small systems would advance at the pace of
+// the costliest group, and a small request's latency would grow with what it shares the
+// session with.  A group repeats its chunk about cost_max / cost times (at most
+// kMaxGroupReps), with the cost of one of its iterations modelled as its cells plus a
+// fixed launch cost of kLaunchCells (a fit of single-system FP64 iteration times: about
+// 28 us at 48^3 and 140 us at 128^3 on MI355X).  One group: no repetition.
+constexpr int kMaxGroupReps = 16;
+constexpr double kLaunchCells = 4.0e5;
+void session_group_reps(mx_session* S, const std::vector<int>& sl) {
+  std::vector<double> cost(S->groups.size(), kLaunchCells);
+  double cmax = 0.0;
+  for (size_t g = 0; g < S->groups.size(); ++g) {
+    const Group& G = S->groups[g];
+    for (int i = 0; i < G.ns; ++i) cost[g] += (double)(S->h_sys[sl[G.s_off + i]].n3 / 3);
+    cmax = std::max(cmax, cost[g]);
+  }
+  for (size_t g = 0; g < S->groups.size(); ++g)
+    S->groups[g].reps = std::max(1, std::min(kMaxGroupReps, (int)(cmax / cost[g])));
+}
+
 int session_rebuild(mx_session* S) {
   std::vector<Member> members;
   for (int i = 0; i < S->capacity; ++i)
@@ -1287,6 +1311,7 @@ int session_rebuild(mx_session* S) {
   std::vector<BlockMap> a, v;
   std::vector<int> sl;
   make_groups(members, S->o.mode, a, v, sl, S->groups);
+  session_group_reps(S, sl);
   S->nsl = (int)sl.size();
   S->nb_apply = (int)a.size();
   S->nb_vec = (int)v.size();

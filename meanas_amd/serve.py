@@ -3,7 +3,9 @@
 Requests (the arguments of meanas.fdfd.solvers.generic, forward or adjoint; reference
 meanas/fdfd/solvers.py:66-141) arrive over time.  The server keeps a session of slots on
 the device (`mx_session_*`, C ABI): every step runs `chunk` COCG iterations on all
-active slots in one set of launches; finished slots are read out and retired and
+active slots, launch group by launch group, with groups of smaller systems repeating
+their chunk (about the same time per group, up to 16 chunks), so a small request is not
+held to the pace of a large one; finished slots are read out and retired and
 waiting requests are admitted as soon as a slot and enough HBM are free, so new work
 joins while other solves are in flight.  A request's result does not depend on what
 else is in the session: each system's block partition and reduction order depend on
@@ -19,7 +21,7 @@ Sessions are kept per lane, a (precision, iteration form) pair.  In mixed precis
 (complex64 iterations, FP64 residual replacement) a request whose complex64 iterations
 stall or break down continues in an FP64 session from its best FP64 iterate, as
 `solve_batch` does; its result carries fp64_fallback and the summed iteration counts.
-FP64 sessions run the fused iteration form (two launches per iteration); a request
+FP64 sessions run the fused iteration form (three launches per iteration); a request
 whose fused recursion loses track finishes there with MX_FLAG_SWITCHED and continues
 from its iterate in a classic-form FP64 session, as `mx_solve` does, with the summed
 iteration counts.

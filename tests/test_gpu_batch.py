@@ -274,6 +274,44 @@ def test_server_alone_equals_mixed(method):
             assert physical_residual(r.omega, r.dxes, r.epsilon, r.J, m.E) < 1e-8
 
 
+def test_session_steps_give_small_groups_equal_time():
+    """A session step gives each launch group about the same time: the chunk of a group of
+    small systems repeats (about cost_max / cost times, its cells plus a fixed launch cost
+    of 4e5 cells).  A 16x14x12 request (real isotropic eps) beside a 96^3 one (per-component
+    eps: another material kind, so another group) advances 3 chunks per step, the large one
+    1, and the small request's field has the bits it has alone."""
+    from meanas_amd import _lib, workloads
+    from meanas_amd.serve import Request, Server
+    g = numpy.random.default_rng(11)
+    omega = 2 * numpy.pi / 12.0
+    reqs = []
+    for tag, shape, aniso in (('small', (16, 14, 12), False), ('large', (96, 96, 96), True)):
+        dxes = workloads.pml_dxes(shape, omega, thickness=4)
+        if aniso:
+            eps = workloads.aniso_structure(shape, g, nbox=5).ravel()
+        else:
+            eps = numpy.tile(workloads.paint(shape, 1.0, workloads.random_boxes(g, shape, 3, 1.5, 4.0)).ravel(), 3)
+        J = workloads.gaussian_sheet(shape, shape[0] // 2, 2.0, workloads.random_polarisation(g)).ravel()
+        reqs.append(Request(omega=omega, dxes=dxes, J=J, epsilon=eps, rtol=1e-9, arrival=0.0, tag=tag))
+    srv = Server(capacity=2, mode=_lib.MX_FP64, chunk=8)
+    idx = {r.tag: srv.admit(srv.prepare(r), 0.0) for r in reqs}
+    srv.step()
+    stats, _ = srv.poll(srv.lane)
+    srv.close()
+    assert stats[idx['large']].iterations == 8
+    assert stats[idx['small']].iterations == 8 * 3
+    alone = Server(capacity=1, mode=_lib.MX_FP64, chunk=8)
+    a = alone.run([reqs[0]], realtime=False)[0]
+    alone.close()
+    srv = Server(capacity=2, mode=_lib.MX_FP64, chunk=8)
+    both = {res.tag: res for res in srv.run(reqs, realtime=False)}
+    srv.close()
+    assert both['small'].stats['status'] == 'converged' and both['large'].stats['status'] == 'converged'
+    assert both['small'].stats['iterations'] == a.stats['iterations']
+    assert numpy.array_equal(both['small'].E, a.E)
+    assert physical_residual(reqs[0].omega, reqs[0].dxes, reqs[0].epsilon, reqs[0].J, a.E) < 1e-8
+
+
 def _near_static_request(tag, adjoint=False):
     """The reference's near-static fixture (meanas/test/test_fdfd.py, omega = 1/1500): its
     fused FP64 recursion loses track and the solve continues in the classic form."""
