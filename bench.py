@@ -645,9 +645,13 @@ def main():
         log(f'fp64 reference solve (1e-11): {st_ref}; field error of the measured solve {ferr:.3e}')
         t_step = float(numpy.median(lat))
         t_fp64, t_mixed = (t_step, t2) if step_mode == _lib.MX_FP64 else (t2, t_step)
+        from meanas_amd.fdfd.autotune import tuned
+        ty2, slab2 = tuned(op, other)     # the tiling that solve ran with (its iteration count depends on it)
         extra.update({'comparison': dict(mode='mixed' if other == _lib.MX_MIXED else 'fp64', solves_per_s=1.0 / t2,
                                          ms=t2 * 1e3, iterations=st2['iterations'], rel_residual=st2['rel_residual'],
-                                         status=st2['status'], fp64_fallback=bool(st2.get('fp64_fallback', False))),
+                                         status=st2['status'], fp64_fallback=bool(st2.get('fp64_fallback', False)),
+                                         mixed_iterations=st2.get('mixed_iterations'),
+                                         tiling=dict(tile_rows=ty2, slab=slab2)),
                       'mixed_over_fp64_speedup': t_fp64 / t_mixed,
                       'field_error_vs_fp64_rtol_1e-11': ferr})
         del x2
