@@ -293,3 +293,17 @@ def test_server_phase_counts_across_hand_overs():
     # a classic FP64 continuation of a complex64 phase (fused form disabled)
     st = merge_prior(dict(ph[2]), hand_over_prior(None, ph[0], mixed, 'classic'), classic)
     assert st == dict(ph[2], iterations=125, replacements=3, fp64_fallback=True, mixed_iterations=70)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: `--gpus N` with N > 1 (or a WORLD_SIZE above 1) exits
+    with status 2 and says so, before touching a device."""
+    import subprocess
+    import sys
+    env = dict(os.environ)
+    for argv, extra in ((['--gpus', '2'], {}), ([], {'WORLD_SIZE': '4'})):
+        env2 = dict(env, **extra)
+        res = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), *argv], capture_output=True, text=True,
+                             env=env2, timeout=120)
+        assert res.returncode == 2, res.stderr
+        assert 'one GPU only' in res.stdout
