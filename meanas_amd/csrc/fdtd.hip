@@ -31,6 +31,11 @@ struct RGrid {
 };
 
 __device__ __forceinline__ double width(const double* w, int n) { return w ? w[n] : 1.0; }
+// width n of a table with n uniform over the block: a scalar load (constant address
+// space) into SGPRs
+__device__ __forceinline__ double width_u(const double* w, int n) {
+  return w ? ((const MX_CONST double*)w)[n] : 1.0;
+}
 
 template <typename T>
 __global__ __launch_bounds__(kFB) void k_fdtd_h(RGrid g, T dt, const T* __restrict__ e, T* __restrict__ h,
@@ -241,10 +246,18 @@ __global__ __launch_bounds__(64 * kStepTY) void k_fdtd_step(RGrid g, T dt, const
     fetch(B, ib);    // E(ib+1), H(ib), mu(ib), eps(ib)
   }
 
+  // float32: the x widths of the next plane are read one step ahead with scalar loads
+  // (as vector loads at the top of the step they are waited for right there, a memory
+  // round trip per plane): 246 against 255-257 us per 256^3 step.  float64 keeps the
+  // vector loads: ahead of time its step measured 432 against 427 us, and carried in the
+  // prefetch bundle the widths took it over 128 VGPRs (a wave per SIMD less, 504 us).
+  constexpr bool kAheadWidths = sizeof(T) == 4;
+  double wxe_n = kAheadWidths ? width_u(g.dxe[0], ib) : 0.0, wxh_n = kAheadWidths ? width_u(g.dxh[0], ib) : 0.0;
 #pragma unroll 1
   for (int i = ib; i < ie; ++i) {
     __syncthreads();   // E(i) is in LDS; the previous step's reads of sH are done
-    const T dxe_i = (T)width(g.dxe[0], i), dxh_i = (T)width(g.dxh[0], i);
+    const T dxe_i = (T)(kAheadWidths ? wxe_n : width(g.dxe[0], i));
+    const T dxh_i = (T)(kAheadWidths ? wxh_n : width(g.dxh[0], i));
     const T e_own[3] = {sE[0][r][q], sE[1][r][q], sE[2][r][q]};
     const T epsv[3] = {B.ep[0], B.ep[1], B.ep[2]};
     // H'(i): own cell
@@ -290,6 +303,12 @@ __global__ __launch_bounds__(64 * kStepTY) void k_fdtd_step(RGrid g, T dt, const
       }
       hpy = hy;
       hpz = hz;
+    }
+    // the next plane's widths, at the end of the step: scalar loads share the wait
+    // counter of LDS reads, so issued at the top they made its first LDS read wait
+    if (kAheadWidths && i + 1 < ie) {
+      wxe_n = width_u(g.dxe[0], i + 1);
+      wxh_n = width_u(g.dxh[0], i + 1);
     }
   }
 }
