@@ -181,7 +181,11 @@ int mx_map(const mx_operator* op, int32_t map, double c_re, double c_im, const v
  * maps, stream, pinned buffers, captured iteration graph; a few KB plus the graph) and
  * reuses it when the next call passes the same operators, systems (rtol, atol and
  * maxiter aside) and options: no allocation, upload or capture then.  Environment
- * MEANAS_AMD_PLAN_CACHE=0 builds every plan afresh. */
+ * MEANAS_AMD_PLAN_CACHE=0 builds every plan afresh.
+ * Several host threads may call it at once.  No entry point issues work on the legacy
+ * (null) stream; a caller's own work on it while another thread's mx_solve captures its
+ * iteration graph is refused by HIP and invalidates that capture, so threads that solve
+ * at once queue their work on streams of their own. */
 int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int32_t nsys,
              const mx_options* opt, mx_stats* stats, void* stream);
 /* Number of mx_solve calls of the calling thread that reused a kept plan (diagnostics;
@@ -189,7 +193,8 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
 int64_t mx_solve_plan_reuses(void);
 /* Free the calling thread's kept plan (its device tables, stream, pinned buffers and
  * graph); no counterpart in the reference.  A thread that ends keeps its plan's memory
- * until the process ends unless it calls this first. */
+ * until the process ends unless it calls this first.  Freeing device memory may
+ * synchronise the device: call it when no other thread is inside mx_solve. */
 int mx_solve_plan_release(void);
 
 /* ---- fields after the solve ------------------------------------------------------

@@ -57,6 +57,30 @@ int fail(const std::string& msg) {
     if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// A copy the host waits for, through a non-blocking stream of the calling thread and
+// device instead of the legacy stream that hipMemcpy uses.  While another thread
+// captures a graph, HIP refuses a legacy-stream operation ("would make the legacy stream
+// depend on a capturing blocking stream"), and the refusal also invalidates that
+// thread's capture, so no entry point issues one.  The streams live until the process
+// ends.  Callers order the copy after device work that produces its source themselves
+// (a stream synchronisation); every destination here is read by work queued afterwards.
+hipError_t copy_now(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  thread_local std::vector<std::pair<int, hipStream_t>> streams;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  hipStream_t s = nullptr;
+  for (const auto& ds : streams)
+    if (ds.first == dev) s = ds.second;
+  if (!s) {
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    streams.push_back({dev, s});
+  }
+  e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 constexpr int kVecBlock = 256;
 constexpr int kVecElemsPerThread = 8;
 #ifndef MX_VEC_MAX_BLOCKS
@@ -355,9 +379,9 @@ int layout_groups(Batch& B, const std::vector<char>& active) {
   B.nb_apply0 = (int)apply.size();
   B.nb_vec0 = (int)vec.size();
   B.nsl0 = (int)sl.size();
-  if (!apply.empty()) MX_TRY(hipMemcpy(B.bm_apply0, apply.data(), sizeof(BlockMap) * apply.size(), hipMemcpyHostToDevice));
-  if (!vec.empty()) MX_TRY(hipMemcpy(B.bm_vec0, vec.data(), sizeof(BlockMap) * vec.size(), hipMemcpyHostToDevice));
-  if (!sl.empty()) MX_TRY(hipMemcpy(B.syslist0, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
+  if (!apply.empty()) MX_TRY(copy_now(B.bm_apply0, apply.data(), sizeof(BlockMap) * apply.size(), hipMemcpyHostToDevice));
+  if (!vec.empty()) MX_TRY(copy_now(B.bm_vec0, vec.data(), sizeof(BlockMap) * vec.size(), hipMemcpyHostToDevice));
+  if (!sl.empty()) MX_TRY(copy_now(B.syslist0, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
   B.use_all();
   return 0;
 }
@@ -439,9 +463,9 @@ int build_batch(Batch& B, const mx_operator* ops, int nops, const mx_system* sys
     if (o.mode == MX_MIXED) v32[i] = view_of<float>(ops[i]);
     else std::memset(&v32[i], 0, sizeof(v32[i]));
   }
-  MX_TRY(hipMemcpy(B.d_ops32, v32.data(), sizeof(OpView<float>) * nops, hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(B.d_ops64, v64.data(), sizeof(OpView<double>) * nops, hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(B.d_sys, B.h_sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
+  MX_TRY(copy_now(B.d_ops32, v32.data(), sizeof(OpView<float>) * nops, hipMemcpyHostToDevice));
+  MX_TRY(copy_now(B.d_ops64, v64.data(), sizeof(OpView<double>) * nops, hipMemcpyHostToDevice));
+  MX_TRY(copy_now(B.d_sys, B.h_sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
   if (layout_groups(B, std::vector<char>(nsys, 1))) return -1;
 
   for (auto& h : B.h_state) MX_TRY(hipHostMalloc(reinterpret_cast<void**>(&h), sizeof(SysState) * nsys, hipHostMallocDefault));
@@ -744,13 +768,13 @@ mx_plan* mx_plan_create(const mx_operator* op, int32_t mode, int32_t tile_rows, 
   if (e != hipSuccess) { fail(std::string("hipMalloc: ") + hipGetErrorString(e)); delete P; return nullptr; }
   if (mode == MX_MIXED) {
     OpView<float> v = view_of<float>(*op);
-    e = hipMemcpy(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
+    e = copy_now(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
   } else {
     OpView<double> v = view_of<double>(*op);
-    e = hipMemcpy(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
+    e = copy_now(P->d_mem, &v, sizeof(v), hipMemcpyHostToDevice);
   }
-  if (e == hipSuccess) e = hipMemcpy(P->d_mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(P->d_mem + s_op + s_sys, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = copy_now(P->d_mem + s_op, &d, sizeof(d), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = copy_now(P->d_mem + s_op + s_sys, bm.data(), sizeof(BlockMap) * bm.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) { fail(std::string("hipMemcpy: ") + hipGetErrorString(e)); (void)hipFree(P->d_mem); delete P; return nullptr; }
   return P;
 }
@@ -1100,7 +1124,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
     MX_TRY(hipStreamSynchronize(s));
     // a system may have finished in the chunk that was still queued: take the newest states
     for (int i = 0; i < nsys; ++i) {
-      MX_TRY(hipMemcpy(&B.h_state[slot][i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
+      MX_TRY(copy_now(&B.h_state[slot][i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
       active[i] = !(B.h_state[slot][i].flags & SF_DONE);
     }
     compacted = true;
@@ -1135,7 +1159,7 @@ int mx_solve(const mx_operator* ops, int32_t nops, const mx_system* systems, int
   MX_TRY(hipStreamSynchronize(s));
   // final states (the queued chunk, if any, was a no-op on finished systems)
   std::vector<SysState> fin(nsys);
-  for (int i = 0; i < nsys; ++i) MX_TRY(hipMemcpy(&fin[i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nsys; ++i) MX_TRY(copy_now(&fin[i], B.h_sys[i].st, sizeof(SysState), hipMemcpyDeviceToHost));
   for (int i = 0; i < nsys; ++i)
     if (wants_best(fin[i]))
       MX_TRY(hipMemcpyAsync(B.h_sys[i].vec[VS_X], B.h_sys[i].vec[VS_XBEST], (size_t)B.h_sys[i].n3 * 16,
@@ -1322,7 +1346,7 @@ int session_rebuild(mx_session* S) {
   std::copy(a.begin(), a.end(), S->h_bm);
   std::copy(v.begin(), v.end(), S->h_bm + a.size());
   if (need) MX_TRY(hipMemcpyAsync(S->d_bm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice, S->stream));
-  if (!sl.empty()) MX_TRY(hipMemcpy(S->d_syslist, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
+  if (!sl.empty()) MX_TRY(copy_now(S->d_syslist, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice));
   if (S->gexec) { MX_TRY(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
   if (S->graph) { MX_TRY(hipGraphDestroy(S->graph)); S->graph = nullptr; }
   S->dirty = false;
@@ -1423,10 +1447,10 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   OpView<float> v32;
   if (S->o.mode == MX_MIXED) v32 = view_of<float>(*op);
   else std::memset(&v32, 0, sizeof(v32));
-  MX_TRY(hipMemcpy(S->d_ops32 + slot, &v32, sizeof(v32), hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(S->d_ops64 + slot, &v64, sizeof(v64), hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(S->d_sys + slot, &d, sizeof(d), hipMemcpyHostToDevice));
-  MX_TRY(hipMemcpy(d.st, st, sizeof(*st), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(S->d_ops32 + slot, &v32, sizeof(v32), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(S->d_ops64 + slot, &v64, sizeof(v64), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(S->d_sys + slot, &d, sizeof(d), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(d.st, st, sizeof(*st), hipMemcpyHostToDevice));
 
   // admission kernels on this slot only: x_lo = 0, ||b||, r = b - A x0
   std::vector<BlockMap> a, v;
@@ -1439,9 +1463,9 @@ int mx_session_admit(mx_session* S, const mx_operator* op, const mx_system* y, v
   if (ensure_staging(S, need)) return -1;
   std::copy(a.begin(), a.end(), S->h_bm);
   std::copy(v.begin(), v.end(), S->h_bm + a.size());
-  MX_TRY(hipMemcpy(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(S->d_adm, S->h_bm, need * sizeof(BlockMap), hipMemcpyHostToDevice));
   int* adm_sl = S->d_syslist + S->capacity;
-  MX_TRY(hipMemcpy(adm_sl, &slot, sizeof(int), hipMemcpyHostToDevice));
+  MX_TRY(copy_now(adm_sl, &slot, sizeof(int), hipMemcpyHostToDevice));
   Batch B = session_batch(S, S->d_adm, (int)a.size(), S->d_adm + a.size(), (int)v.size(), adm_sl, 1, ty, op_kind(*op));
   int rc = 0;
   if (launch_zero_slot(S->o.mode == MX_FP64, B.d_sys, B.d_bmap_vec, B.nb_vec, VS_XLO, s) != hipSuccess) rc = fail("zero x_lo");

@@ -162,10 +162,13 @@ def test_kept_plans_equal_fresh_plans(monkeypatch):
         assert sa == sb
 
 
-def test_kept_plans_are_per_thread():
+@pytest.mark.parametrize('rnd', range(3))   # the threads interleave differently each time
+def test_kept_plans_are_per_thread(rnd):
     """Two host threads solving different systems at the same time, twice each (the
     second call reuses each thread's kept plan), give the bits of the same solves run one
-    after the other."""
+    after the other.  Each thread queues its own torch work on a stream of its own: an
+    operation on the legacy stream while the other thread captures its chunk graph is
+    refused by HIP, and invalidates that capture (the library itself issues none)."""
     import threading
     import torch
     from meanas_amd import _lib
@@ -180,22 +183,34 @@ def test_kept_plans_are_per_thread():
 
     def run(case, x, out, key):
         op, b, work = case
-        for rtol in (1e-6, 1e-9):
-            x.zero_()
-            st = solve_batch([op], [dict(op=0, b=b, x=x, work=work, rtol=rtol)], mode=_lib.MX_FP64)[0]
-            out[(key, rtol)] = (x.clone(), st)
-        torch.cuda.synchronize()
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            for rtol in (1e-6, 1e-9):
+                x.zero_()
+                st = solve_batch([op], [dict(op=0, b=b, x=x, work=work, rtol=rtol)], mode=_lib.MX_FP64,
+                                 stream=stream)[0]
+                out[(key, rtol)] = (x.clone(), st)
+        stream.synchronize()
+
+    done = threading.Barrier(2)
 
     def run_and_release(case, x, out, key):
-        run(case, x, out, key)
-        _lib.check(_lib.lib().mx_solve_plan_release(), 'mx_solve_plan_release')   # the thread ends
+        try:
+            run(case, x, out, key)
+        finally:
+            # the thread ends: release its plan once neither thread captures any more (the
+            # release frees device memory, which may synchronise the device)
+            done.wait(timeout=120)
+            _lib.check(_lib.lib().mx_solve_plan_release(), 'mx_solve_plan_release')
 
     seq = {}
     for k, case in enumerate(cases):
         run(case, torch.zeros_like(case[1]), seq, k)
     par = {}
-    threads = [threading.Thread(target=run_and_release, args=(case, torch.zeros_like(case[1]), par, k))
-               for k, case in enumerate(cases)]
+    xs = [torch.zeros_like(case[1]) for case in cases]
+    torch.cuda.synchronize()            # inputs and outputs ready before the threads start
+    threads = [threading.Thread(target=run_and_release, args=(case, x, par, k))
+               for k, (case, x) in enumerate(zip(cases, xs))]
     for t in threads:
         t.start()
     for t in threads:
